@@ -1,0 +1,684 @@
+// C ABI of the gdem engine: contexts, scenes, stepping, downloads, parity hooks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gdem.h"
+#include "gdem_kernels.h"
+#include "gdem_mesh.h"
+
+struct gdem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace gdem {
+namespace {
+
+int fail(gdem_ctx* c, int code, const std::string& what) {
+  if (c) c->err = what;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? GDEM_E_OOM : GDEM_E_HIP,           \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+// Owns every device allocation of a scene.
+struct DevArena {
+  std::vector<void*> ptrs;
+  ~DevArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t n) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) {
+      ptrs.push_back(p);
+      *out = static_cast<T*>(p);
+    }
+    return e;
+  }
+};
+
+struct SceneBase {
+  virtual ~SceneBase() = default;
+};
+
+}  // namespace
+}  // namespace gdem
+
+struct gdem_scene {
+  gdem_ctx* ctx = nullptr;
+  gdem::SimParams P{};
+  int precision = GDEM_FP64;
+  int graph_steps = 0;
+  int64_t steps = 0;
+  int rebuild_base = 0;
+  gdem::MeshHost mesh;
+  gdem::DevArena arena;
+  gdem::TriGeo* d_tri = nullptr;
+  gdem::BvhNode* d_bvh = nullptr;
+  int* d_ctl = nullptr;
+  unsigned long long* d_acc = nullptr;
+  gdem::Bufs<double> b64{};
+  gdem::Bufs<float> b32{};
+  hipGraphExec_t graph = nullptr;
+  int timing = 0;
+  std::vector<hipEvent_t> ev;  // pairs around force launches
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  double last_total_ms = 0, last_force_ms = 0;
+  int last_force_launches = 0;
+
+  ~gdem_scene() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+  }
+};
+
+namespace gdem {
+namespace {
+
+template <class H>
+int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  auto& A = sc->arena;
+  for (int h = 0; h < 2; ++h) {
+    HIPCHK(c, A.alloc(&B.pos[h], N));
+    HIPCHK(c, A.alloc(&B.vel[h], N));
+    HIPCHK(c, A.alloc(&B.angv[h], N));
+    HIPCHK(c, A.alloc(&B.mat[h], N));
+    HIPCHK(c, A.alloc(&B.pid[h], N));
+    HIPCHK(c, A.alloc(&B.hkey[h], N * sc->P.hist_cap));
+    HIPCHK(c, A.alloc(&B.hd[h], N * sc->P.hist_cap * 3));
+    HIPCHK(c, A.alloc(&B.nh[h], N));
+    HIPCHK(c, A.alloc(&B.wkey[h], N * sc->P.whist_cap));
+    HIPCHK(c, A.alloc(&B.wd[h], N * sc->P.whist_cap * 3));
+    HIPCHK(c, A.alloc(&B.nw[h], N));
+    HIPCHK(c, A.alloc(&B.key[h], N));
+    HIPCHK(c, A.alloc(&B.idx[h], N));
+  }
+  HIPCHK(c, A.alloc(&B.ref, N));
+  HIPCHK(c, A.alloc(&B.nbr, N * sc->P.nb_cap));
+  HIPCHK(c, A.alloc(&B.nnb, N));
+  HIPCHK(c, A.alloc(&B.wl, N * sc->P.wall_cap));
+  HIPCHK(c, A.alloc(&B.nwl, N));
+  HIPCHK(c, A.alloc(&B.cell_tmp, N));
+  HIPCHK(c, A.alloc(&B.cell_of, N));
+  HIPCHK(c, A.alloc(&B.cell_start, ncell));
+  HIPCHK(c, A.alloc(&B.cell_end, ncell));
+  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)256 * sc->P.rs_blocks));
+  HIPCHK(c, hipMemsetAsync(B.cell_start, 0xff, ncell * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.cell_end, 0xff, ncell * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.nh[0], 0, N * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.nw[0], 0, N * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.nnb, 0, N * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.nwl, 0, N * sizeof(int), c->stream));
+  B.tri = sc->d_tri;
+  B.bvh = sc->d_bvh;
+  B.ctl = sc->d_ctl;
+  return GDEM_OK;
+}
+
+template <class H>
+Bufs<H>& bufs(gdem_scene* sc);
+template <>
+Bufs<double>& bufs<double>(gdem_scene* sc) { return sc->b64; }
+template <>
+Bufs<float>& bufs<float>(gdem_scene* sc) { return sc->b32; }
+
+template <class H>
+void enqueue_steps(gdem_scene* sc, int n) {
+  Bufs<H>& B = bufs<H>(sc);
+  hipStream_t st = sc->ctx->stream;
+  for (int i = 0; i < n; ++i) {
+    launch_step<H, H>(sc->P, B, st);
+    launch_rebuild<H>(sc->P, B, st);
+  }
+}
+
+template <class H>
+int step_impl(gdem_scene* sc, int nsteps) {
+  gdem_ctx* c = sc->ctx;
+  hipStream_t st = c->stream;
+  Bufs<H>& B = bufs<H>(sc);
+  if (sc->timing) {
+    // direct launches with an event pair around every force kernel
+    for (auto e : sc->ev) (void)hipEventDestroy(e);
+    sc->ev.assign((size_t)2 * nsteps, nullptr);
+    for (auto& e : sc->ev) HIPCHK(c, hipEventCreate(&e));
+    HIPCHK(c, hipEventRecord(sc->t0, st));
+    for (int i = 0; i < nsteps; ++i) {
+      HIPCHK(c, hipEventRecord(sc->ev[2 * i], st));
+      launch_step<H, H>(sc->P, B, st);
+      HIPCHK(c, hipEventRecord(sc->ev[2 * i + 1], st));
+      launch_rebuild<H>(sc->P, B, st);
+    }
+    HIPCHK(c, hipEventRecord(sc->t1, st));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(sc->t1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, sc->t0, sc->t1));
+    sc->last_total_ms = ms;
+    double fsum = 0;
+    for (int i = 0; i < nsteps; ++i) {
+      float m = 0;
+      HIPCHK(c, hipEventElapsedTime(&m, sc->ev[2 * i], sc->ev[2 * i + 1]));
+      fsum += m;
+    }
+    sc->last_force_ms = fsum;
+    sc->last_force_launches = nsteps;
+    sc->steps += nsteps;
+    return GDEM_OK;
+  }
+  int left = nsteps;
+  if (sc->graph_steps > 0 && nsteps >= sc->graph_steps) {
+    if (!sc->graph) {
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue_steps<H>(sc, sc->graph_steps);
+      HIPCHK(c, hipStreamEndCapture(st, &g));
+      hipError_t e = hipGraphInstantiate(&sc->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(c, e);
+    }
+    while (left >= sc->graph_steps) {
+      HIPCHK(c, hipGraphLaunch(sc->graph, st));
+      left -= sc->graph_steps;
+    }
+  }
+  enqueue_steps<H>(sc, left);
+  HIPCHK(c, hipGetLastError());
+  sc->steps += nsteps;
+  return GDEM_OK;
+}
+
+template <class H>
+int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
+  gdem_ctx* c = sc->ctx;
+  const int N = sc->P.N;
+  size_t ncell = (size_t)sc->P.size[0] * sc->P.size[1] * sc->P.size[2];
+  Bufs<H>& B = bufs<H>(sc);
+  int rc = alloc_bufs<H>(sc, B, ncell);
+  if (rc) return rc;
+  std::vector<double4> pos(N), vel(N), angv(N), mat(N);
+  std::vector<int> pid(N);
+  for (int i = 0; i < N; ++i) {
+    const double r = pd->r[i], m = pd->m[i];
+    const bool act = pd->is_active ? pd->is_active[i] == 1 : true;
+    pos[i] = make_double4(pd->x[3 * i], pd->x[3 * i + 1], pd->x[3 * i + 2], act ? r : -r);
+    const double* v = pd->v ? pd->v + 3 * i : nullptr;
+    const double* w = pd->angv ? pd->angv + 3 * i : nullptr;
+    // ParticleSystem.cu:294-335: invm = 1/m, sqrtm = sqrt(m), moi = 2/5 m r^2
+    vel[i] = make_double4(v ? v[0] : 0., v ? v[1] : 0., v ? v[2] : 0., 1. / m);
+    angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., (double)i);
+    const double rsq = r * r;
+    const double moi = 2. / 5. * m * rsq;
+    mat[i] = make_double4(pd->k[i], pd->etaconst[i], sqrt(m), 1. / moi);
+    pid[i] = i;
+  }
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipMemcpyAsync(B.pos[0], pos.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B.vel[0], vel.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B.angv[0], angv.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B.mat[0], mat.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B.pid[0], pid.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
+  int ctl[CTL_WORDS] = {0};
+  ctl[CTL_REFRESH] = 1;  // the initial lists are built like main.cu:151,196 do on the host
+  HIPCHK(c, hipMemcpyAsync(sc->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
+  launch_rebuild<H>(sc->P, B, st);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+  int rb = 0;
+  HIPCHK(c, hipMemcpy(&rb, sc->d_ctl + CTL_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
+  sc->rebuild_base = rb;
+  return GDEM_OK;
+}
+
+template <class H>
+int download_impl(gdem_scene* sc, gdem_particles_out* out) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  DevArena tmp;
+  double *x, *v, *w;
+  int* a;
+  HIPCHK(c, tmp.alloc(&x, 3 * N));
+  HIPCHK(c, tmp.alloc(&v, 3 * N));
+  HIPCHK(c, tmp.alloc(&w, 3 * N));
+  HIPCHK(c, tmp.alloc(&a, N));
+  launch_download<H>(sc->P, bufs<H>(sc), x, v, w, a, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out->x) HIPCHK(c, hipMemcpy(out->x, x, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->v) HIPCHK(c, hipMemcpy(out->v, v, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->angv) HIPCHK(c, hipMemcpy(out->angv, w, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->is_active) HIPCHK(c, hipMemcpy(out->is_active, a, N * 4, hipMemcpyDeviceToHost));
+  return GDEM_OK;
+}
+
+template <class H>
+int debug_impl(gdem_scene* sc, gdem_debug_view* vw) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  const SimParams& P = sc->P;
+  DevArena tmp;
+  DebugDev D{};
+  HIPCHK(c, tmp.alloc(&D.cellx, 3 * N));
+  HIPCHK(c, tmp.alloc(&D.cell_id, N));
+  HIPCHK(c, tmp.alloc(&D.morton, N));
+  HIPCHK(c, tmp.alloc(&D.nb, N * P.nb_cap));
+  HIPCHK(c, tmp.alloc(&D.nnb, N));
+  HIPCHK(c, tmp.alloc(&D.wall, N * P.wall_cap));
+  HIPCHK(c, tmp.alloc(&D.nwall, N));
+  HIPCHK(c, tmp.alloc(&D.hist_key, N * P.hist_cap));
+  HIPCHK(c, tmp.alloc(&D.hist_delta, N * P.hist_cap * 3));
+  HIPCHK(c, tmp.alloc(&D.nhist, N));
+  HIPCHK(c, tmp.alloc(&D.whist_key, N * P.whist_cap));
+  HIPCHK(c, tmp.alloc(&D.whist_delta, N * P.whist_cap * 3));
+  HIPCHK(c, tmp.alloc(&D.nwhist, N));
+  HIPCHK(c, hipMemsetAsync(D.nb, 0xff, N * P.nb_cap * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(D.wall, 0xff, N * P.wall_cap * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(D.hist_key, 0xff, N * P.hist_cap * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(D.whist_key, 0xff, N * P.whist_cap * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(D.hist_delta, 0, N * P.hist_cap * 24, c->stream));
+  HIPCHK(c, hipMemsetAsync(D.whist_delta, 0, N * P.whist_cap * 24, c->stream));
+  launch_debug<H>(P, bufs<H>(sc), D, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (dst) HIPCHK(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GDEM_OK;
+  };
+  int rc = 0;
+  rc |= cp(vw->cellx, D.cellx, 3 * N * 4);
+  rc |= cp(vw->cell_id, D.cell_id, N * 4);
+  rc |= cp(vw->morton, D.morton, N * 4);
+  rc |= cp(vw->nb, D.nb, N * P.nb_cap * 4);
+  rc |= cp(vw->nnb, D.nnb, N * 4);
+  rc |= cp(vw->wall, D.wall, N * P.wall_cap * 4);
+  rc |= cp(vw->nwall, D.nwall, N * 4);
+  rc |= cp(vw->hist_key, D.hist_key, N * P.hist_cap * 4);
+  rc |= cp(vw->hist_delta, D.hist_delta, N * P.hist_cap * 24);
+  rc |= cp(vw->nhist, D.nhist, N * 4);
+  rc |= cp(vw->whist_key, D.whist_key, N * P.whist_cap * 4);
+  rc |= cp(vw->whist_delta, D.whist_delta, N * P.whist_cap * 24);
+  rc |= cp(vw->nwhist, D.nwhist, N * 4);
+  return rc ? GDEM_E_HIP : GDEM_OK;
+}
+
+template <class H>
+int record_impl(gdem_scene* sc, gdem_contact_view* vw) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  const int cap = vw->rec_cap;
+  DevArena tmp;
+  RecDev R{};
+  R.cap = cap;
+  HIPCHK(c, tmp.alloc(&R.count, N));
+  HIPCHK(c, tmp.alloc(&R.key, N * cap));
+  HIPCHK(c, tmp.alloc(&R.kind, N * cap));
+  HIPCHK(c, tmp.alloc(&R.fn, N * cap * 3));
+  HIPCHK(c, tmp.alloc(&R.ft, N * cap * 3));
+  HIPCHK(c, tmp.alloc(&R.delta, N * cap));
+  HIPCHK(c, hipMemsetAsync(R.count, 0, N * 4, c->stream));
+  launch_step_record<H, H>(sc->P, bufs<H>(sc), R, c->stream);
+  launch_rebuild<H>(sc->P, bufs<H>(sc), c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  sc->steps += 1;
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (dst) HIPCHK(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GDEM_OK;
+  };
+  int rc = 0;
+  rc |= cp(vw->count, R.count, N * 4);
+  rc |= cp(vw->key, R.key, N * cap * 4);
+  rc |= cp(vw->kind, R.kind, N * cap * 4);
+  rc |= cp(vw->fn, R.fn, N * cap * 24);
+  rc |= cp(vw->ft, R.ft, N * cap * 24);
+  rc |= cp(vw->delta, R.delta, N * cap * 8);
+  return rc ? GDEM_E_HIP : GDEM_OK;
+}
+
+template <class H>
+int stats_impl(gdem_scene* sc, gdem_stats* out) {
+  gdem_ctx* c = sc->ctx;
+  HIPCHK(c, hipMemsetAsync(sc->d_acc, 0, 4 * sizeof(unsigned long long), c->stream));
+  launch_counts<H>(sc->P, bufs<H>(sc), sc->d_acc, c->stream);
+  HIPCHK(c, hipGetLastError());
+  unsigned long long acc[4];
+  int ctl[CTL_WORDS];
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(acc, sc->d_acc, sizeof(acc), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ctl, sc->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+  out->steps = sc->steps;
+  out->rebuilds = ctl[CTL_REBUILDS] - sc->rebuild_base;
+  out->n_active = (int)acc[0];
+  out->errors = (uint32_t)ctl[CTL_ERR];
+  const double na = acc[0] ? (double)acc[0] : 1.0;
+  out->mean_nb = acc[1] / na;
+  out->mean_wall = acc[2] / na;
+  out->mean_contacts = acc[3] / na;
+  return GDEM_OK;
+}
+
+}  // namespace
+}  // namespace gdem
+
+using namespace gdem;
+
+extern "C" {
+
+int gdem_ctx_create(int hip_device, gdem_ctx** out) {
+  if (!out) return GDEM_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GDEM_E_HIP;
+  if (hip_device < 0 || hip_device >= ndev) return GDEM_E_ARG;
+  auto* c = new gdem_ctx();
+  c->device = hip_device;
+  if (hipSetDevice(hip_device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GDEM_E_HIP;
+  }
+  *out = c;
+  return GDEM_OK;
+}
+
+void gdem_ctx_destroy(gdem_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int gdem_sync(gdem_ctx* c) {
+  if (!c) return GDEM_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GDEM_OK;
+}
+
+const char* gdem_last_error(gdem_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gdem_material(int n, const double* r, double density, double young, double cor, double* m_out,
+                  double* k_out, double* eta_out) {
+  if (n < 0 || !r || !m_out || !k_out || !eta_out || !(cor > 0.0)) return GDEM_E_ARG;
+  for (int i = 0; i < n; ++i) {
+    const double ri = r[i];
+    const double m = density * 3.14 * ri * ri * ri * 4. / 3.;
+    const double k = 4. / 3. * (young * 0.5) * sqrt(ri * 0.5) * sqrt(0.05 * ri);
+    m_out[i] = m;
+    k_out[i] = k;
+    eta_out[i] = -2. * log(cor) * sqrt(k / (3.1415 * 3.1415 + log(cor) * log(cor)));
+  }
+  return GDEM_OK;
+}
+
+int gdem_bbox_rule(const gdem_mesh_desc* mesh, const double inlet[6], double bbox[6]) {
+  if (!inlet || !bbox) return GDEM_E_ARG;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  if (mesh && mesh->ntri > 0) {
+    for (int t = 0; t < mesh->ntri; ++t)
+      for (int v = 0; v < 3; ++v)
+        for (int a = 0; a < 3; ++a) {
+          const double q = mesh->verts[t * 9 + v * 3 + a];
+          lo[a] = q < lo[a] ? q : lo[a];
+          hi[a] = q > hi[a] ? q : hi[a];
+        }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double mn = lo[a] < inlet[2 * a] ? lo[a] : inlet[2 * a];
+    const double mx = hi[a] > inlet[2 * a + 1] ? hi[a] : inlet[2 * a + 1];
+    bbox[2 * a] = mn * 1.1;
+    bbox[2 * a + 1] = mx * 1.1;
+  }
+  return GDEM_OK;
+}
+
+int gdem_params_default(double max_r, gdem_params* p) {
+  if (!p || !(max_r > 0)) return GDEM_E_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->cell = max_r * 4.;
+  p->skin = max_r;
+  p->refresh = max_r * 0.5;
+  p->precision = GDEM_FP64;
+  return GDEM_OK;
+}
+
+int gdem_mesh_precompute(const gdem_mesh_desc* md, double* geo, int* ids, int* order) {
+  if (!md || md->ntri <= 0 || !md->verts) return GDEM_E_ARG;
+  MeshHost mh;
+  if (!build_mesh(md->verts, md->ntri, md->tris_per_file, md->nfiles, &mh)) return GDEM_E_ARG;
+  for (int t = 0; t < md->ntri; ++t) {
+    const TriGeo& g = mh.tri[t];
+    if (geo) std::memcpy(geo + (size_t)t * 29, &g, 29 * sizeof(double));
+    if (ids) {
+      const int v[6] = {g.v0, g.v1, g.v2, g.e0, g.e1, g.e2};
+      std::memcpy(ids + (size_t)t * 6, v, sizeof(v));
+    }
+    if (order) order[t] = mh.order[t];
+  }
+  return GDEM_OK;
+}
+
+int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mesh_desc* md,
+                      const gdem_params* pp, gdem_scene** out) {
+  if (!c || !pd || !pp || !out) return GDEM_E_ARG;
+  *out = nullptr;
+  if (pd->n <= 0 || !pd->x || !pd->r || !pd->m || !pd->k || !pd->etaconst)
+    return fail(c, GDEM_E_ARG, "particles: n, x, r, m, k and etaconst are required");
+  if (!(pp->dt > 0) || !(pp->cell > 0) || !(pp->skin >= 0) || !(pp->refresh >= 0))
+    return fail(c, GDEM_E_ARG, "params: dt, cell must be > 0, skin and refresh >= 0");
+  if (pp->precision != GDEM_FP64 && pp->precision != GDEM_MIXED)
+    return fail(c, GDEM_E_ARG, "params: unknown precision");
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, GDEM_E_HIP, "hipSetDevice");
+  std::unique_ptr<gdem_scene> sc(new gdem_scene());
+  sc->ctx = c;
+  SimParams& P = sc->P;
+  P.N = pd->n;
+  P.dt = pp->dt;
+  P.mu = pp->mu;
+  double maxr = 0;
+  for (int i = 0; i < pd->n; ++i) maxr = pd->r[i] > maxr ? pd->r[i] : maxr;
+  for (int a = 0; a < 3; ++a) {
+    P.g[a] = pp->g[a];
+    P.bmin[a] = pp->bbox[2 * a];
+    P.bmax[a] = pp->bbox[2 * a + 1];
+    if (!(P.bmax[a] > P.bmin[a])) return fail(c, GDEM_E_ARG, "params: empty bounding box");
+    P.invd[a] = 1. / pp->cell;
+    // BoundingBox.cu:394-396
+    const double cells = ceil((P.bmax[a] - P.bmin[a]) / pp->cell) + 2;
+    if (cells > 1e6) return fail(c, GDEM_E_ARG, "params: cell grid too fine");
+    P.size[a] = (int)cells;
+  }
+  const double ncell = (double)P.size[0] * P.size[1] * P.size[2];
+  if (ncell > 1.5e9) return fail(c, GDEM_E_ARG, "params: cell grid has too many cells");
+  if (pp->cell < 2 * maxr + pp->skin)
+    return fail(c, GDEM_E_ARG, "params: cell must be >= 2*max(r) + skin");
+  P.use_morton = (P.size[0] <= 1024 && P.size[1] <= 1024 && P.size[2] <= 1024) ? 1 : 0;
+  P.skin = pp->skin;
+  P.threshSq = pp->refresh * pp->refresh;
+  P.nb_cap = pp->nb_cap > 0 ? pp->nb_cap : 48;
+  P.wall_cap = pp->wall_cap > 0 ? pp->wall_cap : 24;
+  P.hist_cap = pp->hist_cap > 0 ? pp->hist_cap : 16;
+  P.whist_cap = pp->whist_cap > 0 ? pp->whist_cap : 8;
+  if (P.nb_cap > 64 || P.wall_cap > 48 || P.hist_cap > 64 || P.whist_cap > 64)
+    return fail(c, GDEM_E_ARG, "params: slot capacities exceed 64 (pairs/histories) or 48 (walls)");
+  P.rs_blocks = std::max(1, std::min(1024, (P.N + 2047) / 2048));
+  P.rs_chunk = ((P.N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
+  sc->precision = pp->precision;
+  sc->graph_steps = pp->graph_steps > 0 ? pp->graph_steps : 0;
+  // every active particle must lie in the box (the cell search assumes it)
+  for (int i = 0; i < pd->n; ++i) {
+    if (pd->is_active && pd->is_active[i] != 1) continue;
+    if (!(pd->r[i] > 0)) return fail(c, GDEM_E_ARG, "particles: radius must be > 0");
+    for (int a = 0; a < 3; ++a) {
+      const double q = pd->x[3 * i + a];
+      if (!(q >= P.bmin[a] && q <= P.bmax[a]))
+        return fail(c, GDEM_E_ARG, "particles: active particle outside the bounding box");
+    }
+  }
+  // mesh
+  const int ntri = md ? md->ntri : 0;
+  if (ntri > 0 && !md->verts) return fail(c, GDEM_E_ARG, "mesh: verts missing");
+  if (!build_mesh(ntri > 0 ? md->verts : nullptr, ntri, md ? md->tris_per_file : nullptr,
+                  md ? md->nfiles : 0, &sc->mesh))
+    return fail(c, GDEM_E_ARG, "mesh: tris_per_file does not add up to ntri");
+  P.ntri = ntri;
+  P.nnode = (int)sc->mesh.nodes.size();
+  HIPCHK(c, sc->arena.alloc(&sc->d_tri, std::max(ntri, 1)));
+  HIPCHK(c, sc->arena.alloc(&sc->d_bvh, std::max(P.nnode, 1)));
+  HIPCHK(c, sc->arena.alloc(&sc->d_ctl, CTL_WORDS));
+  HIPCHK(c, sc->arena.alloc(&sc->d_acc, 4));
+  if (ntri > 0) {
+    HIPCHK(c, hipMemcpy(sc->d_tri, sc->mesh.tri.data(), ntri * sizeof(TriGeo), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(sc->d_bvh, sc->mesh.nodes.data(), P.nnode * sizeof(BvhNode), hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipEventCreate(&sc->t0));
+  HIPCHK(c, hipEventCreate(&sc->t1));
+  int rc = sc->precision == GDEM_FP64 ? create_impl<double>(sc.get(), pd) : create_impl<float>(sc.get(), pd);
+  if (rc) return rc;
+  *out = sc.release();
+  return GDEM_OK;
+}
+
+int gdem_scene_step(gdem_scene* sc, int nsteps) {
+  if (!sc || nsteps < 0) return GDEM_E_ARG;
+  if (nsteps == 0) return GDEM_OK;
+  return sc->precision == GDEM_FP64 ? step_impl<double>(sc, nsteps) : step_impl<float>(sc, nsteps);
+}
+
+int gdem_scene_status(gdem_scene* sc, gdem_stats* st) {
+  if (!sc) return GDEM_E_ARG;
+  gdem_stats tmp;
+  gdem_stats* o = st ? st : &tmp;
+  int rc = sc->precision == GDEM_FP64 ? stats_impl<double>(sc, o) : stats_impl<float>(sc, o);
+  if (rc) return rc;
+  if (o->errors & (GDEM_ERR_NB_OVERFLOW | GDEM_ERR_WALL_OVERFLOW | GDEM_ERR_HIST_OVERFLOW |
+                   GDEM_ERR_WHIST_OVERFLOW | GDEM_ERR_VE_OVERFLOW))
+    return fail(sc->ctx, GDEM_E_SLOT_OVERFLOW, "slot overflow (raise the capacities in gdem_params)");
+  if (o->errors & GDEM_ERR_NAN) return fail(sc->ctx, GDEM_E_NAN, "non-finite particle state");
+  return GDEM_OK;
+}
+
+int gdem_scene_download(gdem_scene* sc, gdem_particles_out* out) {
+  if (!sc || !out) return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? download_impl<double>(sc, out) : download_impl<float>(sc, out);
+}
+
+void gdem_scene_destroy(gdem_scene* sc) {
+  if (!sc) return;
+  (void)hipStreamSynchronize(sc->ctx->stream);
+  delete sc;
+}
+
+int gdem_scene_debug(gdem_scene* sc, gdem_debug_view* vw) {
+  if (!sc || !vw) return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? debug_impl<double>(sc, vw) : debug_impl<float>(sc, vw);
+}
+
+int gdem_scene_step_record(gdem_scene* sc, gdem_contact_view* vw) {
+  if (!sc || !vw || vw->rec_cap <= 0) return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? record_impl<double>(sc, vw) : record_impl<float>(sc, vw);
+}
+
+int gdem_scene_set_timing(gdem_scene* sc, int on) {
+  if (!sc) return GDEM_E_ARG;
+  sc->timing = on ? 1 : 0;
+  return GDEM_OK;
+}
+
+int gdem_scene_timing(gdem_scene* sc, double* total_ms, double* force_ms, int* launches) {
+  if (!sc) return GDEM_E_ARG;
+  if (total_ms) *total_ms = sc->last_total_ms;
+  if (force_ms) *force_ms = sc->last_force_ms;
+  if (launches) *launches = sc->last_force_launches;
+  return GDEM_OK;
+}
+
+int gdem_radix_sort(gdem_ctx* c, int n, const uint32_t* keys, uint32_t* keys_out, int* order_out) {
+  if (!c || n < 0 || (n > 0 && (!keys || !keys_out || !order_out))) return GDEM_E_ARG;
+  if (n == 0) return GDEM_OK;
+  DevArena tmp;
+  uint32_t *k0, *k1, *hist;
+  int *i0, *i1, *ctl;
+  const int blocks = std::max(1, std::min(1024, (n + 2047) / 2048));
+  const int chunk = ((n + blocks - 1) / blocks + 255) / 256 * 256;
+  HIPCHK(c, tmp.alloc(&k0, n));
+  HIPCHK(c, tmp.alloc(&k1, n));
+  HIPCHK(c, tmp.alloc(&i0, n));
+  HIPCHK(c, tmp.alloc(&i1, n));
+  HIPCHK(c, tmp.alloc(&hist, (size_t)256 * blocks));
+  HIPCHK(c, tmp.alloc(&ctl, CTL_WORDS));
+  std::vector<int> iota(n);
+  for (int i = 0; i < n; ++i) iota[i] = i;
+  HIPCHK(c, hipMemcpy(k0, keys, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(i0, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  launch_radix_sort(n, blocks, chunk, k0, i0, k1, i1, hist, ctl, false, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(keys_out, k0, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(order_out, i0, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return GDEM_OK;
+}
+
+int gdem_tri_distance(gdem_ctx* c, const gdem_mesh_desc* md, int tri, int np, const double* pts,
+                      const double* rsq, double* out, int* hit) {
+  if (!c || !md || md->ntri <= 0 || tri < 0 || tri >= md->ntri || np <= 0 || !pts || !rsq || !out || !hit)
+    return GDEM_E_ARG;
+  MeshHost mh;
+  if (!build_mesh(md->verts, md->ntri, md->tris_per_file, md->nfiles, &mh)) return GDEM_E_ARG;
+  DevArena tmp;
+  TriGeo* t;
+  double *p, *r, *o;
+  int* h;
+  HIPCHK(c, tmp.alloc(&t, 1));
+  HIPCHK(c, tmp.alloc(&p, (size_t)np * 3));
+  HIPCHK(c, tmp.alloc(&r, (size_t)np));
+  HIPCHK(c, tmp.alloc(&o, (size_t)np * 4));
+  HIPCHK(c, tmp.alloc(&h, (size_t)np));
+  HIPCHK(c, hipMemcpy(t, &mh.tri[tri], sizeof(TriGeo), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(p, pts, (size_t)np * 24, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(r, rsq, (size_t)np * 8, hipMemcpyHostToDevice));
+  launch_tri_probe(t, np, p, r, o, h, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out, o, (size_t)np * 32, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hit, h, (size_t)np * 4, hipMemcpyDeviceToHost));
+  return GDEM_OK;
+}
+
+int gdem_fp64_probe(gdem_ctx* c, int n, const double* a, const double* b, double* sq, double* dv) {
+  if (!c || n <= 0 || !a || !b || !sq || !dv) return GDEM_E_ARG;
+  DevArena tmp;
+  double *da, *db, *ds, *dd;
+  HIPCHK(c, tmp.alloc(&da, n));
+  HIPCHK(c, tmp.alloc(&db, n));
+  HIPCHK(c, tmp.alloc(&ds, n));
+  HIPCHK(c, tmp.alloc(&dd, n));
+  HIPCHK(c, hipMemcpy(da, a, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(db, b, (size_t)n * 8, hipMemcpyHostToDevice));
+  launch_fp64_probe(n, da, db, ds, dd, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(sq, ds, (size_t)n * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(dv, dd, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return GDEM_OK;
+}
+
+}  // extern "C"

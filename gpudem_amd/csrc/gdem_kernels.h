@@ -1,0 +1,62 @@
+// Launch interface between the host engine and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gdem_types.h"
+
+#define GDEM_ERR_NB_OVERFLOW_BIT 1u
+#define GDEM_ERR_WALL_OVERFLOW_BIT 2u
+#define GDEM_ERR_HIST_OVERFLOW_BIT 4u
+#define GDEM_ERR_WHIST_OVERFLOW_BIT 8u
+#define GDEM_ERR_VE_OVERFLOW_BIT 16u
+#define GDEM_ERR_NAN_BIT 32u
+
+namespace gdem {
+
+struct RecDev {
+  int cap;
+  int* count;
+  int* key;
+  int* kind;
+  double* fn;
+  double* ft;
+  double* delta;
+};
+
+struct DebugDev {
+  int* cellx;
+  int* cell_id;
+  uint32_t* morton;
+  int* nb;
+  int* nnb;
+  int* wall;
+  int* nwall;
+  int* hist_key;
+  double* hist_delta;
+  int* nhist;
+  int* whist_key;
+  double* whist_delta;
+  int* nwhist;
+};
+
+template <class T, class H>
+void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st);
+template <class T, class H>
+void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st);
+template <class H>
+void launch_rebuild(const SimParams& P, const Bufs<H>& B, hipStream_t st);
+template <class H>
+void launch_download(const SimParams& P, const Bufs<H>& B, double* x, double* v, double* w, int* a,
+                     hipStream_t st);
+template <class H>
+void launch_debug(const SimParams& P, const Bufs<H>& B, const DebugDev& D, hipStream_t st);
+template <class H>
+void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc, hipStream_t st);
+
+void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1,
+                       uint32_t* hist, const int* ctl, bool gated, hipStream_t st);
+void launch_tri_probe(const TriGeo* tri, int np, const double* pts, const double* rsq, double* out,
+                      int* hit, hipStream_t st);
+void launch_fp64_probe(int n, const double* a, const double* b, double* sq, double* dv, hipStream_t st);
+
+}  // namespace gdem

@@ -1,0 +1,97 @@
+// Device-side data layout of the gdem engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gdem {
+
+// One triangle, indexed by its original (file-order) id; 256 B so a wave's
+// gathers stay line-aligned.  Fields are the precompute of
+// TriangleMesh.cu:536-677: vertices in ascending merged-vertex-id order,
+// edge vectors, barycentric dot products and their inverses, plane.
+struct alignas(256) TriGeo {
+  double n[3];
+  double d;  // plane offset
+  double p0[3], p1[3], p2[3];
+  double e01[3], e02[3], e12[3];
+  double d00, d01, d11, denom, d00inv, d11inv, d22inv;
+  int v0, v1, v2;  // merged vertex ids (hitAt of a vertex contact)
+  int e0, e1, e2;  // edge ids + nShift (hitAt of an edge contact)
+};
+static_assert(sizeof(TriGeo) == 256, "TriGeo must be 256 B");
+
+// Bounding-volume node with both children's boxes inline: one 128 B load per
+// traversal step.  child >= 0 is an internal node, child < 0 a leaf holding
+// triangle ~child.
+struct alignas(128) BvhNode {
+  double lo[2][3];
+  double hi[2][3];
+  int child[2];
+  int pad[6];
+};
+static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 B");
+
+// Per-scene constants passed by value to every kernel.
+struct SimParams {
+  double dt, mu;
+  double g[3];
+  double bmin[3], bmax[3];
+  double invd[3];
+  double skin, threshSq;
+  int size[3];
+  int use_morton;  // sort by morton3D of the cell (all axes <= 1024 cells) or by cell id
+  int N;
+  int nb_cap, wall_cap, hist_cap, whist_cap;
+  int ntri, nnode;
+  int rs_blocks, rs_chunk;  // radix sort grid and per-block chunk
+};
+
+// Control words shared by all kernels of a scene (device memory).
+enum Ctl : int {
+  CTL_CUR = 0,       // which state buffer is current (flips every step)
+  CTL_HCUR = 1,      // which material/history buffer is current (flips at re-sort)
+  CTL_REFRESH = 2,   // set by the step kernel when a list rebuild is due
+  CTL_ERR = 3,       // GDEM_ERR_* bits
+  CTL_DONE_STEP = 4, // last-block counters
+  CTL_DONE_PERM = 5,
+  CTL_NACT = 6,      // active particles after the last re-sort
+  CTL_DONE_WALL = 7,
+  CTL_REBUILDS = 8,
+  CTL_WORDS = 16
+};
+
+// All device buffers of a scene.  Passed by value; which half of a [2]
+// array is current is read from the control words at kernel start.
+template <class H>
+struct Bufs {
+  double4* pos[2];   // x, y, z, r   (r < 0 marks an inactive particle)
+  double4* vel[2];   // vx, vy, vz, 1/m
+  double4* angv[2];  // wx, wy, wz, unused
+  double4* mat[2];   // k, etaconst, sqrt(m), 1/I
+  int* pid[2];       // original particle id
+  int* hkey[2];      // pair history keys, slot-major [hist_cap][N]
+  H* hd[2];          // pair history deltas, slot-major [hist_cap][3][N]
+  int* nh[2];
+  int* wkey[2];      // wall history keys [whist_cap][N]
+  H* wd[2];          // [whist_cap][3][N]
+  int* nw[2];
+  double4* ref;      // positions at the last list build
+  int* nbr;          // pair candidates (internal index), slot-major [nb_cap][N]
+  int* nnb;
+  int* wl;           // wall candidates (triangle id) [wall_cap][N]
+  int* nwl;
+  uint32_t* key[2];  // radix sort ping-pong
+  int* idx[2];
+  int* cell_tmp;     // cell id per particle before the permutation
+  int* cell_of;      // cell id per particle after it
+  int* cell_start;   // [ncell], validated through cell_of
+  int* cell_end;
+  uint32_t* rs_hist; // [256 * rs_blocks]
+  const TriGeo* tri;
+  const BvhNode* bvh;
+  int* ctl;
+};
+
+}  // namespace gdem

@@ -1,0 +1,338 @@
+"""ctypes binding of libgdem.so (the C ABI declared in include/gdem.h).
+
+The classes mirror the reference's host-side objects: ``Scene`` stands for
+ParticleSys<DeviceMemory> + BoundingBox + BVH + DeviceTriangleMesh after
+``copyToDevice`` (ParticleSystem.cu:82-89) and ``Scene.step`` for repeated
+``device_dem_verlet_verlet`` (device_dem.cu:1195-1236).  There is no CPU
+fallback: without the built library every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdem.so")
+
+GDEM_FP64 = 0
+GDEM_MIXED = 1
+ERRORS = {-1: "GDEM_E_ARG", -2: "GDEM_E_HIP", -3: "GDEM_E_OOM", -4: "GDEM_E_SLOT_OVERFLOW",
+          -5: "GDEM_E_NAN", -6: "GDEM_E_STATE"}
+
+# exported symbols (checked by the CPU test suite against include/gdem.h)
+SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_material",
+           "gdem_bbox_rule", "gdem_params_default", "gdem_scene_create", "gdem_scene_step",
+           "gdem_scene_status", "gdem_scene_download", "gdem_scene_destroy", "gdem_scene_debug",
+           "gdem_scene_step_record", "gdem_scene_timing", "gdem_scene_set_timing",
+           "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute"]
+
+
+class GdemError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("ntri", ctypes.c_int), ("verts", ctypes.c_void_p), ("nfiles", ctypes.c_int),
+                ("tris_per_file", ctypes.c_void_p)]
+
+
+class ParticlesDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("x", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("angv", ctypes.c_void_p), ("r", ctypes.c_void_p), ("m", ctypes.c_void_p),
+                ("k", ctypes.c_void_p), ("etaconst", ctypes.c_void_p),
+                ("is_active", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("mu", ctypes.c_double), ("g", ctypes.c_double * 3),
+                ("bbox", ctypes.c_double * 6), ("cell", ctypes.c_double),
+                ("skin", ctypes.c_double), ("refresh", ctypes.c_double),
+                ("precision", ctypes.c_int), ("nb_cap", ctypes.c_int), ("wall_cap", ctypes.c_int),
+                ("hist_cap", ctypes.c_int), ("whist_cap", ctypes.c_int),
+                ("graph_steps", ctypes.c_int)]
+
+
+class ParticlesOut(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("angv", ctypes.c_void_p),
+                ("is_active", ctypes.c_void_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("rebuilds", ctypes.c_int64), ("n_active", ctypes.c_int),
+                ("errors", ctypes.c_uint32), ("mean_nb", ctypes.c_double),
+                ("mean_wall", ctypes.c_double), ("mean_contacts", ctypes.c_double)]
+
+
+class DebugView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("cellx", "cell_id", "morton", "nb", "nnb", "wall", "nwall", "hist_key",
+                 "hist_delta", "nhist", "whist_key", "whist_delta", "nwhist")]
+
+
+class ContactView(ctypes.Structure):
+    _fields_ = [("rec_cap", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in
+                                              ("count", "key", "kind", "fn", "ft", "delta")]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libgdem.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with gpudem_amd/_build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp = ctypes.c_void_p
+        L.gdem_ctx_create.argtypes = [ctypes.c_int, P(vp)]
+        L.gdem_ctx_destroy.argtypes = [vp]
+        L.gdem_ctx_destroy.restype = None
+        L.gdem_sync.argtypes = [vp]
+        L.gdem_last_error.argtypes = [vp]
+        L.gdem_last_error.restype = ctypes.c_char_p
+        L.gdem_material.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, vp, vp, vp]
+        L.gdem_bbox_rule.argtypes = [P(MeshDesc), vp, vp]
+        L.gdem_params_default.argtypes = [ctypes.c_double, P(Params)]
+        L.gdem_scene_create.argtypes = [vp, P(ParticlesDesc), P(MeshDesc), P(Params), P(vp)]
+        L.gdem_scene_step.argtypes = [vp, ctypes.c_int]
+        L.gdem_scene_status.argtypes = [vp, P(Stats)]
+        L.gdem_scene_download.argtypes = [vp, P(ParticlesOut)]
+        L.gdem_scene_destroy.argtypes = [vp]
+        L.gdem_scene_destroy.restype = None
+        L.gdem_scene_debug.argtypes = [vp, P(DebugView)]
+        L.gdem_scene_step_record.argtypes = [vp, P(ContactView)]
+        L.gdem_scene_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
+        L.gdem_scene_set_timing.argtypes = [vp, ctypes.c_int]
+        L.gdem_radix_sort.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.gdem_tri_distance.argtypes = [vp, P(MeshDesc), ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
+        L.gdem_fp64_probe.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.gdem_mesh_precompute.argtypes = [P(MeshDesc), vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def material(r, density, young, cor):
+    """(m, k, etaconst) per particle, as main.cu:67 and ParticleSystem.cu:202,301."""
+    r = _f64(r).ravel()
+    m, k, e = np.empty_like(r), np.empty_like(r), np.empty_like(r)
+    rc = lib().gdem_material(len(r), _ptr(r), density, young, cor, _ptr(m), _ptr(k), _ptr(e))
+    if rc:
+        raise GdemError(rc, "gdem_material")
+    return m, k, e
+
+
+def mesh_desc(verts, tris_per_file=None):
+    v = _f64(verts).reshape(-1, 9)
+    tpf = None if tris_per_file is None else np.ascontiguousarray(tris_per_file, dtype=np.int32)
+    d = MeshDesc(v.shape[0], _ptr(v), 0 if tpf is None else len(tpf), _ptr(tpf))
+    d._keep = (v, tpf)
+    return d
+
+
+def mesh_precompute(verts, tris_per_file=None):
+    """Host-side triangle precompute of the library (no GPU needed)."""
+    md = mesh_desc(verts, tris_per_file)
+    geo = np.empty((md.ntri, 29))
+    ids = np.empty((md.ntri, 6), np.int32)
+    order = np.empty(md.ntri, np.int32)
+    rc = lib().gdem_mesh_precompute(ctypes.byref(md), _ptr(geo), _ptr(ids), _ptr(order))
+    if rc:
+        raise GdemError(rc, "gdem_mesh_precompute")
+    names = ["n", "d", "p0", "p1", "p2", "e01", "e02", "e12"]
+    out, o = {}, 0
+    for nm in names:
+        w = 1 if nm == "d" else 3
+        out[nm] = geo[:, o:o + w] if w > 1 else geo[:, o]
+        o += w
+    for i, nm in enumerate(["d00", "d01", "d11", "denom", "d00inv", "d11inv", "d22inv"]):
+        out[nm] = geo[:, 22 + i]
+    out["ids"] = ids
+    out["order"] = order
+    return out
+
+
+def bbox_rule(verts, inlet):
+    """calc_BoundingBoxLimits (BoundingBox.cu:448-478)."""
+    md = mesh_desc(verts if verts is not None else np.zeros((0, 9)))
+    inl = _f64(inlet)
+    out = np.empty(6)
+    rc = lib().gdem_bbox_rule(ctypes.byref(md), _ptr(inl), _ptr(out))
+    if rc:
+        raise GdemError(rc, "gdem_bbox_rule")
+    return out
+
+
+class Context:
+    """One HIP device + stream (gdem_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        rc = lib().gdem_ctx_create(device, ctypes.byref(self.h))
+        if rc:
+            raise GdemError(rc, "gdem_ctx_create (no usable HIP device?)")
+
+    def error(self):
+        return lib().gdem_last_error(self.h).decode()
+
+    def check(self, rc, what):
+        if rc:
+            raise GdemError(rc, f"{what}: {self.error()}")
+
+    def sync(self):
+        self.check(lib().gdem_sync(self.h), "gdem_sync")
+
+    def radix_sort(self, keys):
+        k = np.ascontiguousarray(keys, dtype=np.uint32)
+        ko = np.empty_like(k)
+        io = np.empty(len(k), np.int32)
+        self.check(lib().gdem_radix_sort(self.h, len(k), _ptr(k), _ptr(ko), _ptr(io)), "radix_sort")
+        return ko, io
+
+    def tri_distance(self, verts, tri, points, rsq):
+        md = mesh_desc(verts)
+        p = _f64(points).reshape(-1, 3)
+        r = _f64(rsq).reshape(-1)
+        out = np.empty((len(p), 4))
+        hit = np.empty(len(p), np.int32)
+        self.check(lib().gdem_tri_distance(self.h, ctypes.byref(md), int(tri), len(p), _ptr(p), _ptr(r),
+                                           _ptr(out), _ptr(hit)), "tri_distance")
+        return out, hit
+
+    def fp64_probe(self, a, b):
+        a = _f64(a)
+        b = _f64(b)
+        s = np.empty_like(a)
+        d = np.empty_like(a)
+        self.check(lib().gdem_fp64_probe(self.h, len(a), _ptr(a), _ptr(b), _ptr(s), _ptr(d)), "fp64_probe")
+        return s, d
+
+    def close(self):
+        if self.h:
+            lib().gdem_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Device-resident particle system + mesh (gdem_scene).
+
+    x: (n,3) positions; r: (n,) radii; m, k, etaconst: material per particle
+    (see ``material``); verts: (ntri,9) triangle soup; params: dict with dt, mu,
+    g, bbox, cell, skin, refresh and optional precision, caps, graph_steps.
+    """
+
+    def __init__(self, ctx, x, r, m, k, etaconst, verts, params, v=None, angv=None, active=None,
+                 tris_per_file=None):
+        self.ctx = ctx
+        x = _f64(x, (-1, 3))
+        self.n = x.shape[0]
+        arrs = dict(x=x, r=_f64(r).ravel(), m=_f64(m).ravel(), k=_f64(k).ravel(),
+                    e=_f64(etaconst).ravel(),
+                    v=None if v is None else _f64(v, (-1, 3)),
+                    w=None if angv is None else _f64(angv, (-1, 3)),
+                    a=None if active is None else np.ascontiguousarray(active, dtype=np.int32))
+        pd = ParticlesDesc(self.n, _ptr(arrs["x"]), _ptr(arrs["v"]), _ptr(arrs["w"]), _ptr(arrs["r"]),
+                           _ptr(arrs["m"]), _ptr(arrs["k"]), _ptr(arrs["e"]), _ptr(arrs["a"]))
+        md = mesh_desc(verts if verts is not None else np.zeros((0, 9)), tris_per_file)
+        p = Params()
+        p.dt = params["dt"]
+        p.mu = params["mu"]
+        p.g = (ctypes.c_double * 3)(*params["g"])
+        p.bbox = (ctypes.c_double * 6)(*params["bbox"])
+        p.cell = params["cell"]
+        p.skin = params["skin"]
+        p.refresh = params["refresh"]
+        p.precision = params.get("precision", GDEM_FP64)
+        p.nb_cap = params.get("nb_cap", 0)
+        p.wall_cap = params.get("wall_cap", 0)
+        p.hist_cap = params.get("hist_cap", 0)
+        p.whist_cap = params.get("whist_cap", 0)
+        p.graph_steps = params.get("graph_steps", 0)
+        self.caps = dict(nb=p.nb_cap or 48, wall=p.wall_cap or 24, hist=p.hist_cap or 16,
+                         whist=p.whist_cap or 8)
+        self.h = ctypes.c_void_p()
+        ctx.check(lib().gdem_scene_create(ctx.h, ctypes.byref(pd), ctypes.byref(md), ctypes.byref(p),
+                                          ctypes.byref(self.h)), "gdem_scene_create")
+
+    def step(self, n=1):
+        self.ctx.check(lib().gdem_scene_step(self.h, int(n)), "gdem_scene_step")
+
+    def status(self, check=True):
+        st = Stats()
+        rc = lib().gdem_scene_status(self.h, ctypes.byref(st))
+        if check:
+            self.ctx.check(rc, "gdem_scene_status")
+        return dict(steps=st.steps, rebuilds=st.rebuilds, n_active=st.n_active, errors=st.errors,
+                    mean_nb=st.mean_nb, mean_wall=st.mean_wall, mean_contacts=st.mean_contacts, rc=rc)
+
+    def download(self):
+        x = np.empty((self.n, 3))
+        v = np.empty((self.n, 3))
+        w = np.empty((self.n, 3))
+        a = np.empty(self.n, np.int32)
+        out = ParticlesOut(_ptr(x), _ptr(v), _ptr(w), _ptr(a))
+        self.ctx.check(lib().gdem_scene_download(self.h, ctypes.byref(out)), "gdem_scene_download")
+        return dict(x=x, v=v, angv=w, active=a)
+
+    def debug(self):
+        n, c = self.n, self.caps
+        d = dict(cellx=np.empty((n, 3), np.int32), cell_id=np.empty(n, np.int32),
+                 morton=np.empty(n, np.uint32), nb=np.empty((n, c["nb"]), np.int32),
+                 nnb=np.empty(n, np.int32), wall=np.empty((n, c["wall"]), np.int32),
+                 nwall=np.empty(n, np.int32), hist_key=np.empty((n, c["hist"]), np.int32),
+                 hist_delta=np.empty((n, c["hist"], 3)), nhist=np.empty(n, np.int32),
+                 whist_key=np.empty((n, c["whist"]), np.int32),
+                 whist_delta=np.empty((n, c["whist"], 3)), nwhist=np.empty(n, np.int32))
+        view = DebugView(*[_ptr(d[f]) for f, _ in DebugView._fields_])
+        self.ctx.check(lib().gdem_scene_debug(self.h, ctypes.byref(view)), "gdem_scene_debug")
+        return d
+
+    def step_record(self, cap=64):
+        n = self.n
+        d = dict(count=np.zeros(n, np.int32), key=np.empty((n, cap), np.int32),
+                 kind=np.empty((n, cap), np.int32), fn=np.empty((n, cap, 3)), ft=np.empty((n, cap, 3)),
+                 delta=np.empty((n, cap)))
+        view = ContactView(cap, *[_ptr(d[f]) for f in ("count", "key", "kind", "fn", "ft", "delta")])
+        self.ctx.check(lib().gdem_scene_step_record(self.h, ctypes.byref(view)), "gdem_scene_step_record")
+        return d
+
+    def set_timing(self, on=True):
+        self.ctx.check(lib().gdem_scene_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def timing(self):
+        t = ctypes.c_double()
+        f = ctypes.c_double()
+        k = ctypes.c_int()
+        self.ctx.check(lib().gdem_scene_timing(self.h, ctypes.byref(t), ctypes.byref(f), ctypes.byref(k)),
+                       "timing")
+        return t.value, f.value, k.value
+
+    def close(self):
+        if self.h:
+            lib().gdem_scene_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,206 @@
+/*
+ * gdem -- MI355X-native DEM contact engine, C ABI.
+ *
+ * This is the drop-in boundary for the per-step sphere contact path of
+ * bi-rugaoishii/gpudem.  Each entry point names the reference interface it
+ * replaces.  All pointers are plain host pointers borrowed for the duration of
+ * the call; device memory is owned by the library.  Every function returns
+ * GDEM_OK (0) or a negative GDEM_E_* code; no exceptions cross the ABI.
+ * A context is single-threaded (callers synchronise externally); distinct
+ * contexts are independent.
+ */
+#ifndef GDEM_H_
+#define GDEM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDEM_OK 0
+#define GDEM_E_ARG (-1)           /* invalid argument or shape */
+#define GDEM_E_HIP (-2)           /* HIP runtime error (see gdem_last_error) */
+#define GDEM_E_OOM (-3)           /* device allocation failed or over budget */
+#define GDEM_E_SLOT_OVERFLOW (-4) /* a neighbour / contact slot row overflowed */
+#define GDEM_E_NAN (-5)           /* non-finite particle state detected */
+#define GDEM_E_STATE (-6)         /* call not valid in the object's state */
+
+/* precision of the contact arithmetic */
+#define GDEM_FP64 0  /* plain FP64, bit-identical to the reference */
+#define GDEM_MIXED 1 /* FP32 forces/history, FP64 relative position, overlap and integration */
+
+/* bits of the error word reported by gdem_scene_status */
+#define GDEM_ERR_NB_OVERFLOW 1u
+#define GDEM_ERR_WALL_OVERFLOW 2u
+#define GDEM_ERR_HIST_OVERFLOW 4u
+#define GDEM_ERR_WHIST_OVERFLOW 8u
+#define GDEM_ERR_VE_OVERFLOW 16u
+#define GDEM_ERR_NAN 32u
+
+typedef struct gdem_ctx gdem_ctx;
+typedef struct gdem_scene gdem_scene;
+
+/* Triangle soup as read from one or more ASCII STL files: facet vertices in
+ * file order.  Vertices are merged per file on a 1e-9 grid and edge ids are
+ * global, as TriangleMesh.cu:363-752 does. */
+typedef struct {
+  int ntri;
+  const double* verts;      /* [ntri*9]: v0xyz v1xyz v2xyz of each facet */
+  int nfiles;               /* 0 means one file */
+  const int* tris_per_file; /* [nfiles] or NULL */
+} gdem_mesh_desc;
+
+/* Host particle arrays in the reference's ParticleSys<HostMemory> layout
+ * (ParticleSystemMember_common.def): xyz triples are AoS [n*3]. */
+typedef struct {
+  int n;
+  const double* x;        /* [n*3] positions */
+  const double* v;        /* [n*3] velocities, NULL = zero */
+  const double* angv;     /* [n*3] angular velocities, NULL = zero */
+  const double* r;        /* [n] radii */
+  const double* m;        /* [n] masses */
+  const double* k;        /* [n] normal stiffness */
+  const double* etaconst; /* [n] damping constant */
+  const int* is_active;   /* [n], NULL = all active */
+} gdem_particles_desc;
+
+/* Step parameters.  The reference derives cell = 4*max r, skin = max r,
+ * refresh = 0.5*max r (BoundingBox.cu:381-388); gdem_params_default fills
+ * exactly that. */
+typedef struct {
+  double dt;
+  double mu;        /* Coulomb friction coefficient */
+  double g[3];      /* gravity */
+  double bbox[6];   /* minx, maxx, miny, maxy, minz, maxz */
+  double cell;      /* cell-grid spacing */
+  double skin;      /* Verlet skin for pair and wall candidates */
+  double refresh;   /* displacement that triggers a list rebuild */
+  int precision;    /* GDEM_FP64 or GDEM_MIXED */
+  int nb_cap;       /* pair candidates per particle (0 = 48) */
+  int wall_cap;     /* wall candidates per particle (0 = 24) */
+  int hist_cap;     /* pair contact history slots (0 = 16) */
+  int whist_cap;    /* wall contact history slots (0 = 8) */
+  int graph_steps;  /* steps per captured hipGraph (0 = no capture) */
+} gdem_params;
+
+/* Host download view (original particle order). Any pointer may be NULL. */
+typedef struct {
+  double* x;      /* [n*3] */
+  double* v;      /* [n*3] */
+  double* angv;   /* [n*3] */
+  int* is_active; /* [n] */
+} gdem_particles_out;
+
+/* Run statistics. */
+typedef struct {
+  int64_t steps;
+  int64_t rebuilds;
+  int n_active;
+  uint32_t errors;     /* GDEM_ERR_* bits */
+  double mean_nb;      /* mean pair-candidate list length over active particles */
+  double mean_wall;    /* mean wall-candidate list length */
+  double mean_contacts;/* mean live pair + wall history slots per active particle */
+} gdem_stats;
+
+/* Debug views (original particle ids / triangle ids) for parity checks. */
+typedef struct {
+  int* cellx;           /* [n*3] cell coordinates as d_calcCellId computes them */
+  int* cell_id;         /* [n] linear cell id */
+  uint32_t* morton;     /* [n] morton3D of the cell coordinates (10-bit masks) */
+  int* nb;              /* [n*nb_cap] pair candidates as original ids, ascending */
+  int* nnb;             /* [n] */
+  int* wall;            /* [n*wall_cap] wall candidates (triangle ids), ascending */
+  int* nwall;           /* [n] */
+  int* hist_key;        /* [n*hist_cap] pair history keys in slot order */
+  double* hist_delta;   /* [n*hist_cap*3] */
+  int* nhist;           /* [n] */
+  int* whist_key;       /* [n*whist_cap] */
+  double* whist_delta;  /* [n*whist_cap*3] */
+  int* nwhist;          /* [n] */
+} gdem_debug_view;
+
+/* Per-contact record of one step (pairs ascending by partner id, then walls
+ * ascending by triangle id): [n*rec_cap] rows. */
+typedef struct {
+  int rec_cap;
+  int* count;    /* [n] */
+  int* key;      /* partner original id or triangle id */
+  int* kind;     /* 0 pair, 1 wall */
+  double* fn;    /* [n*rec_cap*3] normal force */
+  double* ft;    /* [n*rec_cap*3] tangential force */
+  double* delta; /* [n*rec_cap] overlap */
+} gdem_contact_view;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the implicit CUDA context + legacy stream of main.cu:206-245. */
+int gdem_ctx_create(int hip_device, gdem_ctx** out);
+void gdem_ctx_destroy(gdem_ctx* ctx);
+int gdem_sync(gdem_ctx* ctx);
+const char* gdem_last_error(gdem_ctx* ctx);
+
+/* ---- set-up helpers ----------------------------------------------------- */
+/* Material of one particle: m = density*3.14*r^3*4/3 (main.cu:67),
+ * k = 4/3*(E/2)*sqrt(r/2)*sqrt(0.05 r) (ParticleSystem.cu:202),
+ * etaconst = -2 ln(e) sqrt(k/(3.1415^2+ln(e)^2)) (ParticleSystem.cu:301). */
+int gdem_material(int n, const double* r, double density, double young, double cor,
+                  double* m_out, double* k_out, double* etaconst_out);
+/* calc_BoundingBoxLimits (BoundingBox.cu:448-478): min/max of the mesh and
+ * inlet boxes, each scaled by 1.1. */
+int gdem_bbox_rule(const gdem_mesh_desc* mesh, const double inlet[6], double bbox_out[6]);
+/* cell = 4 max r, skin = max r, refresh = 0.5 max r (BoundingBox.cu:381-388). */
+int gdem_params_default(double max_r, gdem_params* p);
+
+/* Host-only triangle precompute (TriangleMesh.cu:536-677), for inspection and
+ * parity checks; needs no GPU.  geo_out[ntri*29] holds per triangle
+ * n[3], d, p0[3], p1[3], p2[3], e01[3], e02[3], e12[3], d00, d01, d11, denom,
+ * d00inv, d11inv, d22inv; ids_out[ntri*6] holds the vertex ids v0<v1<v2 and
+ * the edge ids e01, e12, e02 (+3*ntri); order_out[ntri] the Morton order of
+ * the centroids (stable, as radixSortUint32 leaves it). Any output may be NULL. */
+int gdem_mesh_precompute(const gdem_mesh_desc* mesh, double* geo_out, int* ids_out, int* order_out);
+
+/* ---- plain scene --------------------------------------------------------- */
+/* Replaces allocate(ParticleSys<DeviceMemory>*) + copyToDevice
+ * (ParticleSystem.h:78,91), initialize_BoundingBox + copyToDeviceBox
+ * (BoundingBox.h:160,179), initializeBVH/buildBVH/copyToDeviceBVH (BVH.h:47-62),
+ * deviceMallocCopyTriangleMesh (TriangleMesh.h:184) and the host list builds of
+ * main.cu:151,196.  Copies in; the caller keeps its host arrays. */
+int gdem_scene_create(gdem_ctx* ctx, const gdem_particles_desc* particles,
+                      const gdem_mesh_desc* mesh, const gdem_params* params, gdem_scene** out);
+/* Replaces nsteps calls of device_dem_verlet_verlet (device_dem.h:112,
+ * device_dem.cu:1195-1236).  Asynchronous on the context stream. */
+int gdem_scene_step(gdem_scene* scene, int nsteps);
+/* Waits, then reports errors raised by the kernels (overflow, NaN). */
+int gdem_scene_status(gdem_scene* scene, gdem_stats* stats);
+/* Replaces copyFromDevice (ParticleSystem.h:99, SAVE_ON members). */
+int gdem_scene_download(gdem_scene* scene, gdem_particles_out* out);
+/* Replaces deallocate(ParticleSys<DeviceMemory>*) and the free_* calls. */
+void gdem_scene_destroy(gdem_scene* scene);
+/* Parity hooks: cell coordinates/keys of the current positions, candidate
+ * lists and contact histories, all in original particle ids. */
+int gdem_scene_debug(gdem_scene* scene, gdem_debug_view* view);
+/* Runs one step that also records every contact's forces. */
+int gdem_scene_step_record(gdem_scene* scene, gdem_contact_view* view);
+/* Device wall-clock of the last gdem_scene_step call's kernels (ms), and of
+ * the force kernel alone, measured with HIP events on the context stream. */
+int gdem_scene_timing(gdem_scene* scene, double* total_ms, double* force_ms, int* force_launches);
+/* Enables (1) or disables (0) HIP-event timing of the force kernel. */
+int gdem_scene_set_timing(gdem_scene* scene, int on);
+
+/* ---- primitives exposed for parity tests ---------------------------------- */
+/* Stable LSD radix sort of (key, iota) on the device: the order
+ * cub::DeviceRadixSort::SortPairs gives (BoundingBox.cu:1235-1242). */
+int gdem_radix_sort(gdem_ctx* ctx, int n, const uint32_t* keys, uint32_t* keys_out, int* order_out);
+/* d_dist_triangle (device_dem.cu:13-205) on the device for np points against
+ * one triangle of a mesh: out[np*4] = (nx, ny, nz, dist), hit[np]. */
+int gdem_tri_distance(gdem_ctx* ctx, const gdem_mesh_desc* mesh, int tri, int np,
+                      const double* points, const double* rsq, double* out, int* hit);
+/* Device FP64 sqrt and division on arrays (checks correct rounding). */
+int gdem_fp64_probe(gdem_ctx* ctx, int n, const double* a, const double* b, double* sqrt_out,
+                    double* div_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDEM_H_ */
