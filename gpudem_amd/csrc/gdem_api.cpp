@@ -65,6 +65,7 @@ struct gdem_scene {
   gdem::SimParams P{};
   int precision = GDEM_FP64;
   int graph_steps = 0;
+  bool coop = false;  // rebuild as one cooperative kernel (else the launch chain)
   int64_t steps = 0;
   int rebuild_base = 0;
   gdem::MeshHost mesh;
@@ -122,7 +123,7 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
   HIPCHK(c, A.alloc(&B.cell_of, N));
   HIPCHK(c, A.alloc(&B.cell_start, ncell));
   HIPCHK(c, A.alloc(&B.cell_end, ncell));
-  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)256 * sc->P.rs_blocks));
+  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)256 * sc->P.rs_blocks + 256));
   HIPCHK(c, hipMemsetAsync(B.cell_start, 0xff, ncell * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.cell_end, 0xff, ncell * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nh[0], 0, N * sizeof(int), c->stream));
@@ -143,13 +144,14 @@ template <>
 Bufs<float>& bufs<float>(gdem_scene* sc) { return sc->b32; }
 
 template <class H>
-void enqueue_steps(gdem_scene* sc, int n) {
+int enqueue_steps(gdem_scene* sc, int n) {
   Bufs<H>& B = bufs<H>(sc);
   hipStream_t st = sc->ctx->stream;
   for (int i = 0; i < n; ++i) {
     launch_step<H, H>(sc->P, B, st);
-    launch_rebuild<H>(sc->P, B, st);
+    HIPCHK(sc->ctx, launch_rebuild<H>(sc->P, B, sc->coop, st));
   }
+  return GDEM_OK;
 }
 
 template <class H>
@@ -167,7 +169,7 @@ int step_impl(gdem_scene* sc, int nsteps) {
       HIPCHK(c, hipEventRecord(sc->ev[2 * i], st));
       launch_step<H, H>(sc->P, B, st);
       HIPCHK(c, hipEventRecord(sc->ev[2 * i + 1], st));
-      launch_rebuild<H>(sc->P, B, st);
+      HIPCHK(c, launch_rebuild<H>(sc->P, B, sc->coop, st));
     }
     HIPCHK(c, hipEventRecord(sc->t1, st));
     HIPCHK(c, hipGetLastError());
@@ -191,8 +193,9 @@ int step_impl(gdem_scene* sc, int nsteps) {
     if (!sc->graph) {
       hipGraph_t g = nullptr;
       HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_steps<H>(sc, sc->graph_steps);
+      const int erc = enqueue_steps<H>(sc, sc->graph_steps);
       HIPCHK(c, hipStreamEndCapture(st, &g));
+      if (erc) return erc;
       hipError_t e = hipGraphInstantiate(&sc->graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       HIPCHK(c, e);
@@ -202,7 +205,8 @@ int step_impl(gdem_scene* sc, int nsteps) {
       left -= sc->graph_steps;
     }
   }
-  enqueue_steps<H>(sc, left);
+  const int erc = enqueue_steps<H>(sc, left);
+  if (erc) return erc;
   HIPCHK(c, hipGetLastError());
   sc->steps += nsteps;
   return GDEM_OK;
@@ -214,6 +218,14 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   const int N = sc->P.N;
   size_t ncell = (size_t)sc->P.size[0] * sc->P.size[1] * sc->P.size[2];
   Bufs<H>& B = bufs<H>(sc);
+  // radix-sort / cooperative grid: one workgroup per chunk of >= 256 keys
+  SimParams& P = sc->P;
+  const int want = (N + 255) / 256;
+  const int limit = sc->graph_steps > 0 ? 0 : coop_grid_limit<H>();
+  sc->coop = limit > 0;
+  P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want))
+                         : std::max(1, std::min(1024, (N + 2047) / 2048));
+  P.rs_chunk = ((N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
   int rc = alloc_bufs<H>(sc, B, ncell);
   if (rc) return rc;
   std::vector<double4> pos(N), vel(N), angv(N), mat(N);
@@ -241,8 +253,7 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   int ctl[CTL_WORDS] = {0};
   ctl[CTL_REFRESH] = 1;  // the initial lists are built like main.cu:151,196 do on the host
   HIPCHK(c, hipMemcpyAsync(sc->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
-  launch_rebuild<H>(sc->P, B, st);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_rebuild<H>(sc->P, B, sc->coop, st));
   HIPCHK(c, hipStreamSynchronize(st));
   int rb = 0;
   HIPCHK(c, hipMemcpy(&rb, sc->d_ctl + CTL_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
@@ -337,8 +348,7 @@ int record_impl(gdem_scene* sc, gdem_contact_view* vw) {
   HIPCHK(c, tmp.alloc(&R.delta, N * cap));
   HIPCHK(c, hipMemsetAsync(R.count, 0, N * 4, c->stream));
   launch_step_record<H, H>(sc->P, bufs<H>(sc), R, c->stream);
-  launch_rebuild<H>(sc->P, bufs<H>(sc), c->stream);
-  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, launch_rebuild<H>(sc->P, bufs<H>(sc), sc->coop, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   sc->steps += 1;
   auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -519,8 +529,6 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
   P.whist_cap = pp->whist_cap > 0 ? pp->whist_cap : 8;
   if (P.nb_cap > 64 || P.wall_cap > 48 || P.hist_cap > 64 || P.whist_cap > 64)
     return fail(c, GDEM_E_ARG, "params: slot capacities exceed 64 (pairs/histories) or 48 (walls)");
-  P.rs_blocks = std::max(1, std::min(1024, (P.N + 2047) / 2048));
-  P.rs_chunk = ((P.N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
   sc->precision = pp->precision;
   sc->graph_steps = pp->graph_steps > 0 ? pp->graph_steps : 0;
   // every active particle must lie in the box (the cell search assumes it)

@@ -43,8 +43,14 @@ template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st);
 template <class T, class H>
 void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st);
+// coop: one cooperative kernel (grid = P.rs_blocks, co-resident); otherwise
+// the launch chain, which can be captured into a hipGraph.
 template <class H>
-void launch_rebuild(const SimParams& P, const Bufs<H>& B, hipStream_t st);
+hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipStream_t st);
+// Largest co-resident grid of the cooperative rebuild kernel (0 if the
+// device cannot launch cooperatively).
+template <class H>
+int coop_grid_limit();
 template <class H>
 void launch_download(const SimParams& P, const Bufs<H>& B, double* x, double* v, double* w, int* a,
                      hipStream_t st);

@@ -16,6 +16,7 @@
 // (device_dem.cu:288-499, 13-205), history update d_update_history(_wall)
 // (device_dem.cu:743-790), integration k_integrate (device_dem.cu:1065-1091),
 // dk_checkOoB (:1116-1125) and k_shouldRefreshNeighborList (:1095-1112).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "gdem_kernels.h"
@@ -64,13 +65,16 @@ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
 }
 
 // Last-workgroup detection: returns true in exactly one thread (thread 0 of
-// the last workgroup to finish).  No waiting is involved.
+// the last workgroup to finish).  No waiting is involved.  Callers use it only
+// to flip control words that every workgroup read at its start, so no memory
+// fence is needed (a device-scope fence would write back the L2 of the XCD
+// once per workgroup); results reach the next kernel through the kernel
+// boundary.
 __device__ __forceinline__ bool last_block(int* counter) {
   __shared__ int is_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    const int prev = atomicAdd(counter, 1);
+    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == (int)gridDim.x - 1);
   }
   __syncthreads();
@@ -463,7 +467,8 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
 }
 
 // ---------------------------------------------------------------------------
-// list rebuild chain (each kernel returns at once unless the refresh flag is set)
+// list rebuild (each phase is a per-element device function used both by the
+// launch chain and by the single cooperative rebuild kernel)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool rebuild_due(const int* ctl) {
   return __builtin_amdgcn_readfirstlane(ctl[CTL_REFRESH]) != 0;
@@ -471,11 +476,7 @@ __device__ __forceinline__ bool rebuild_due(const int* ctl) {
 
 // Cell coordinates (d_calcCellId, BoundingBox.cu:1089-1104) and sort keys.
 template <class H>
-__global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
-  if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= P.N) return;
-  const int cur = B.ctl[CTL_CUR];
+__device__ __forceinline__ void keys_one(const SimParams& P, const Bufs<H>& B, int s, int cur) {
   const double4 p = B.pos[cur][s];
   uint32_t key = 0xffffffffu;
   int cid = -1;
@@ -492,72 +493,65 @@ __global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
 }
 
 // ---- stable LSD radix sort, 8-bit digits ----------------------------------
-// Each workgroup owns one contiguous chunk of the array; per-(digit, chunk)
-// counts are scanned digit-major so equal digits keep chunk order, and inside
-// a chunk keys are ranked in arrival order with wave ballots.
+// Workgroup b owns the contiguous chunk [b*chunk, (b+1)*chunk); per-(digit,
+// chunk) counts are scanned digit-major so equal digits keep chunk order, and
+// inside a chunk keys are ranked in arrival order with wave ballots.
 constexpr int RS_THREADS = 256;
 
-template <bool GATED>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_count(int n, int chunk, int shift,
-                                                         const uint32_t* __restrict__ keys,
-                                                         uint32_t* __restrict__ hist, const int* ctl) {
-  if (GATED && !rebuild_due(ctl)) return;
+__device__ __forceinline__ void rs_count_block(int n, int chunk, int shift, const uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ hist, int b, int nb) {
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int beg = blockIdx.x * chunk;
+  const int beg = b * chunk;
   const int end = min(n, beg + chunk);
-  for (int i = beg + threadIdx.x; i < end; i += RS_THREADS)
-    atomicAdd(&cnt[(keys[i] >> shift) & 0xffu], 1u);
+  for (int i = beg + threadIdx.x; i < end; i += RS_THREADS) atomicAdd(&cnt[(keys[i] >> shift) & 0xffu], 1u);
   __syncthreads();
-  hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+  hist[(size_t)threadIdx.x * nb + b] = cnt[threadIdx.x];
+  __syncthreads();
 }
 
-// Exclusive scan of m counters by one workgroup of 1024 threads.
-template <bool GATED>
-__global__ __launch_bounds__(1024) void k_rs_scan(int m, uint32_t* __restrict__ hist, const int* ctl) {
-  if (GATED && !rebuild_due(ctl)) return;
-  __shared__ uint32_t part[1024];
-  const int per = (m + 1023) / 1024;
+// Exclusive scan of m counters in place by one workgroup of RS_THREADS
+// threads; returns the total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* __restrict__ v, int m) {
+  __shared__ uint32_t part[RS_THREADS];
+  const int per = (m + RS_THREADS - 1) / RS_THREADS;
   const int beg = threadIdx.x * per;
   const int end = min(m, beg + per);
   uint32_t sum = 0;
-  for (int i = beg; i < end; ++i) sum += hist[i];
+  for (int i = beg; i < end; ++i) sum += v[i];
   part[threadIdx.x] = sum;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < RS_THREADS; off <<= 1) {
     const uint32_t add = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
     __syncthreads();
     part[threadIdx.x] += add;
     __syncthreads();
   }
-  uint32_t run = part[threadIdx.x] - sum;  // exclusive prefix of this thread's range
+  uint32_t run = part[threadIdx.x] - sum;
   for (int i = beg; i < end; ++i) {
-    const uint32_t c = hist[i];
-    hist[i] = run;
+    const uint32_t c = v[i];
+    v[i] = run;
     run += c;
   }
+  const uint32_t total = part[RS_THREADS - 1];
+  __syncthreads();
+  return total;
 }
 
-template <bool GATED>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int shift,
-                                                           const uint32_t* __restrict__ kin,
-                                                           const int* __restrict__ iin,
-                                                           uint32_t* __restrict__ kout,
-                                                           int* __restrict__ iout,
-                                                           const uint32_t* __restrict__ hist,
-                                                           const int* ctl) {
-  if (GATED && !rebuild_due(ctl)) return;
+// Stable scatter of workgroup b's chunk.  base[d] must hold the global
+// position of the first key with digit d in this chunk.
+__device__ __forceinline__ void rs_scatter_block(int n, int chunk, int shift, const uint32_t* __restrict__ kin,
+                                                 const int* __restrict__ iin, uint32_t* __restrict__ kout,
+                                                 int* __restrict__ iout, uint32_t* base, int b) {
   constexpr int NW = RS_THREADS / 64;
-  __shared__ uint32_t base[256];
   __shared__ uint32_t wcnt[NW][256];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  base[tid] = hist[(size_t)tid * gridDim.x + blockIdx.x];
   for (int w = 0; w < NW; ++w) wcnt[w][tid] = 0;
   __syncthreads();
-  const int beg = blockIdx.x * chunk;
+  const int beg = b * chunk;
   const int end = min(n, beg + chunk);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   for (int r0 = beg; r0 < end; r0 += RS_THREADS) {
@@ -568,9 +562,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int
     const uint32_t d = (k >> shift) & 0xffu;
     uint64_t match = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot((d >> b) & 1u);
-      match &= ((d >> b) & 1u) ? bal : ~bal;
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t bal = __ballot((d >> bit) & 1u);
+      match &= ((d >> bit) & 1u) ? bal : ~bal;
     }
     const int rank = __popcll(match & lt);
     if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(match);
@@ -599,50 +593,36 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int
 // Gather every per-particle array into sorted order (d_swap_ps,
 // BoundingBox.cu:928-1005, but including history rows, stiffness and ids).
 template <class H>
-__global__ __launch_bounds__(256) void k_permute(SimParams P, Bufs<H> B) {
-  if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int cur = B.ctl[CTL_CUR];
-  const int hc = B.ctl[CTL_HCUR];
+__device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B, int s, int cur, int hc) {
   const int N = P.N;
-  if (s < N) {
-    const int src = B.idx[0][s];
-    B.pos[cur ^ 1][s] = B.pos[cur][src];
-    B.vel[cur ^ 1][s] = B.vel[cur][src];
-    B.angv[cur ^ 1][s] = B.angv[cur][src];
-    B.mat[hc ^ 1][s] = B.mat[hc][src];
-    B.pid[hc ^ 1][s] = B.pid[hc][src];
-    B.cell_of[s] = B.cell_tmp[src];
-    const int nh = B.nh[hc][src];
-    B.nh[hc ^ 1][s] = nh;
-    for (int q = 0; q < nh; ++q) {
-      B.hkey[hc ^ 1][(size_t)q * N + s] = B.hkey[hc][(size_t)q * N + src];
-      for (int c = 0; c < 3; ++c)
-        B.hd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.hd[hc][((size_t)q * 3 + c) * N + src];
-    }
-    const int nw = B.nw[hc][src];
-    B.nw[hc ^ 1][s] = nw;
-    for (int q = 0; q < nw; ++q) {
-      B.wkey[hc ^ 1][(size_t)q * N + s] = B.wkey[hc][(size_t)q * N + src];
-      for (int c = 0; c < 3; ++c)
-        B.wd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.wd[hc][((size_t)q * 3 + c) * N + src];
-    }
+  const int src = B.idx[0][s];
+  B.pos[cur ^ 1][s] = B.pos[cur][src];
+  B.vel[cur ^ 1][s] = B.vel[cur][src];
+  B.angv[cur ^ 1][s] = B.angv[cur][src];
+  B.mat[hc ^ 1][s] = B.mat[hc][src];
+  B.pid[hc ^ 1][s] = B.pid[hc][src];
+  B.cell_of[s] = B.cell_tmp[src];
+  const int nh = B.nh[hc][src];
+  B.nh[hc ^ 1][s] = nh;
+  for (int q = 0; q < nh; ++q) {
+    B.hkey[hc ^ 1][(size_t)q * N + s] = B.hkey[hc][(size_t)q * N + src];
+    for (int c = 0; c < 3; ++c)
+      B.hd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.hd[hc][((size_t)q * 3 + c) * N + src];
   }
-  if (last_block(&B.ctl[CTL_DONE_PERM])) {
-    B.ctl[CTL_DONE_PERM] = 0;
-    atomicExch(&B.ctl[CTL_CUR], cur ^ 1);
-    atomicExch(&B.ctl[CTL_HCUR], hc ^ 1);
+  const int nw = B.nw[hc][src];
+  B.nw[hc ^ 1][s] = nw;
+  for (int q = 0; q < nw; ++q) {
+    B.wkey[hc ^ 1][(size_t)q * N + s] = B.wkey[hc][(size_t)q * N + src];
+    for (int c = 0; c < 3; ++c)
+      B.wd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.wd[hc][((size_t)q * 3 + c) * N + src];
   }
 }
 
 // Cell table from the sorted keys: start/end of each occupied cell.  Entries
 // of empty cells are stale; lookups validate them through cell_of.
 template <class H>
-__global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
-  if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cells_one(const SimParams& P, const Bufs<H>& B, int s) {
   const int N = P.N;
-  if (s >= N) return;
   const uint32_t* key = B.key[0];
   const uint32_t k = key[s];
   if (k == 0xffffffffu) {
@@ -662,14 +642,9 @@ __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
 constexpr int NB_MAX = 64;
 
 template <class H>
-__global__ __launch_bounds__(256) void k_pairlist(SimParams P, Bufs<H> B) {
-  if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pairlist_one(const SimParams& P, const Bufs<H>& B, int s, int cur, int hc,
+                                             int nact) {
   const int N = P.N;
-  if (s >= N) return;
-  const int cur = B.ctl[CTL_CUR];
-  const int hc = B.ctl[CTL_HCUR];
-  const int nact = B.ctl[CTL_NACT];
   const double4* __restrict__ pos = B.pos[cur];
   const int* __restrict__ pid = B.pid[hc];
   const double4 p = pos[s];
@@ -734,75 +709,208 @@ __global__ __launch_bounds__(256) void k_pairlist(SimParams P, Bufs<H> B) {
 
 // Wall candidates by BVH traversal (k_update_neighborlist_wall,
 // BVH.cu:493-540): leaf box test against (r+skin)^2, exact test
-// dist < r + skin, ascending triangle ids.  The last workgroup clears the
-// refresh flag, which ends the rebuild.
+// dist < r + skin, ascending triangle ids.
 constexpr int WL_MAX = 48;
 
+template <class H>
+__device__ __forceinline__ void walllist_one(const SimParams& P, const Bufs<H>& B, int s, int cur, int nact) {
+  const int N = P.N;
+  if (s >= nact || P.ntri == 0) {
+    B.nwl[s] = 0;
+    return;
+  }
+  const double4 p = B.pos[cur][s];
+  const double R = p.w + P.skin;
+  const double Rsq = R * R;
+  const int cap = P.wall_cap < WL_MAX ? P.wall_cap : WL_MAX;
+  int list[WL_MAX];
+  int n = 0;
+  bool ovf = false;
+  int stack[64];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const BvhNode* nd = B.bvh + stack[--sp];
+    const int nchild = (nd->child[0] == nd->child[1]) ? 1 : 2;
+    for (int c = 0; c < nchild; ++c) {
+      double ex = 0.0, ey = 0.0, ez = 0.0;
+      if (p.x < nd->lo[c][0]) ex = nd->lo[c][0] - p.x;
+      else if (p.x > nd->hi[c][0]) ex = p.x - nd->hi[c][0];
+      if (p.y < nd->lo[c][1]) ey = nd->lo[c][1] - p.y;
+      else if (p.y > nd->hi[c][1]) ey = p.y - nd->hi[c][1];
+      if (p.z < nd->lo[c][2]) ez = nd->lo[c][2] - p.z;
+      else if (p.z > nd->hi[c][2]) ez = p.z - nd->hi[c][2];
+      if (!((ex * ex + ey * ey + ez * ez) <= Rsq)) continue;
+      const int ch = nd->child[c];
+      if (ch >= 0) {
+        if (sp < 64) stack[sp++] = ch;
+        else ovf = true;
+      } else {
+        const int t = ~ch;
+        const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, Rsq);
+        if (h.dist < p.w + P.skin) {
+          if (n >= cap) {
+            ovf = true;
+            continue;
+          }
+          int at = n;
+          while (at > 0 && list[at - 1] > t) {
+            list[at] = list[at - 1];
+            --at;
+          }
+          list[at] = t;
+          ++n;
+        }
+      }
+    }
+  }
+  for (int q = 0; q < n; ++q) B.wl[(size_t)q * N + s] = list[q];
+  B.nwl[s] = n;
+  if (ovf) atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_WALL_OVERFLOW_BIT);
+}
+
+// ---- the launch chain (one kernel per phase; used under hipGraph capture) ----
+template <class H>
+__global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
+  if (!rebuild_due(B.ctl)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < P.N) keys_one(P, B, s, B.ctl[CTL_CUR]);
+}
+
+template <bool GATED>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_count(int n, int chunk, int shift,
+                                                         const uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ hist, const int* ctl) {
+  if (GATED && !rebuild_due(ctl)) return;
+  rs_count_block(n, chunk, shift, keys, hist, blockIdx.x, gridDim.x);
+}
+
+// Exclusive scan of the whole digit-major count table by one workgroup.
+template <bool GATED>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(int m, uint32_t* __restrict__ hist, const int* ctl) {
+  if (GATED && !rebuild_due(ctl)) return;
+  block_exclusive_scan(hist, m);
+}
+
+template <bool GATED>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int shift,
+                                                           const uint32_t* __restrict__ kin,
+                                                           const int* __restrict__ iin,
+                                                           uint32_t* __restrict__ kout,
+                                                           int* __restrict__ iout,
+                                                           const uint32_t* __restrict__ hist,
+                                                           const int* ctl) {
+  if (GATED && !rebuild_due(ctl)) return;
+  __shared__ uint32_t base[256];
+  base[threadIdx.x] = hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+  __syncthreads();
+  rs_scatter_block(n, chunk, shift, kin, iin, kout, iout, base, blockIdx.x);
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void k_permute(SimParams P, Bufs<H> B) {
+  if (!rebuild_due(B.ctl)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cur = B.ctl[CTL_CUR];
+  const int hc = B.ctl[CTL_HCUR];
+  if (s < P.N) permute_one(P, B, s, cur, hc);
+  if (last_block(&B.ctl[CTL_DONE_PERM])) {
+    B.ctl[CTL_DONE_PERM] = 0;
+    atomicExch(&B.ctl[CTL_CUR], cur ^ 1);
+    atomicExch(&B.ctl[CTL_HCUR], hc ^ 1);
+  }
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
+  if (!rebuild_due(B.ctl)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < P.N) cells_one(P, B, s);
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void k_pairlist(SimParams P, Bufs<H> B) {
+  if (!rebuild_due(B.ctl)) return;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < P.N) pairlist_one(P, B, s, B.ctl[CTL_CUR], B.ctl[CTL_HCUR], B.ctl[CTL_NACT]);
+}
+
+// The last workgroup clears the refresh flag, which ends the rebuild.
 template <class H>
 __global__ __launch_bounds__(256) void k_walllist(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = P.N;
-  if (s < N) {
-    const int nact = B.ctl[CTL_NACT];
-    if (s >= nact || P.ntri == 0) {
-      B.nwl[s] = 0;
-    } else {
-      const int cur = B.ctl[CTL_CUR];
-      const double4 p = B.pos[cur][s];
-      const double R = p.w + P.skin;
-      const double Rsq = R * R;
-      const int cap = P.wall_cap < WL_MAX ? P.wall_cap : WL_MAX;
-      int list[WL_MAX];
-      int n = 0;
-      bool ovf = false;
-      int stack[64];
-      int sp = 0;
-      stack[sp++] = 0;
-      while (sp > 0) {
-        const BvhNode* nd = B.bvh + stack[--sp];
-        const int nchild = (nd->child[0] == nd->child[1]) ? 1 : 2;
-        for (int c = 0; c < nchild; ++c) {
-          double ex = 0.0, ey = 0.0, ez = 0.0;
-          if (p.x < nd->lo[c][0]) ex = nd->lo[c][0] - p.x;
-          else if (p.x > nd->hi[c][0]) ex = p.x - nd->hi[c][0];
-          if (p.y < nd->lo[c][1]) ey = nd->lo[c][1] - p.y;
-          else if (p.y > nd->hi[c][1]) ey = p.y - nd->hi[c][1];
-          if (p.z < nd->lo[c][2]) ez = nd->lo[c][2] - p.z;
-          else if (p.z > nd->hi[c][2]) ez = p.z - nd->hi[c][2];
-          if (!((ex * ex + ey * ey + ez * ez) <= Rsq)) continue;
-          const int ch = nd->child[c];
-          if (ch >= 0) {
-            if (sp < 64) stack[sp++] = ch;
-            else ovf = true;
-          } else {
-            const int t = ~ch;
-            const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, Rsq);
-            if (h.dist < p.w + P.skin) {
-              if (n >= cap) {
-                ovf = true;
-                continue;
-              }
-              int at = n;
-              while (at > 0 && list[at - 1] > t) {
-                list[at] = list[at - 1];
-                --at;
-              }
-              list[at] = t;
-              ++n;
-            }
-          }
-        }
-      }
-      for (int q = 0; q < n; ++q) B.wl[(size_t)q * N + s] = list[q];
-      B.nwl[s] = n;
-      if (ovf) atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_WALL_OVERFLOW_BIT);
-    }
-  }
+  if (s < P.N) walllist_one(P, B, s, B.ctl[CTL_CUR], B.ctl[CTL_NACT]);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
     atomicExch(&B.ctl[CTL_REFRESH], 0);
+  }
+}
+
+// ---- the whole rebuild in one cooperative kernel ---------------------------
+// Launched every step with a co-resident grid; returns at once (before any
+// grid barrier) when the refresh flag is clear, so an idle step costs one
+// launch.  Every workgroup reads the control words before the first barrier;
+// workgroup 0 updates them after the last one.
+template <class H>
+__global__ __launch_bounds__(256) void k_rebuild_coop(SimParams P, Bufs<H> B) {
+  if (!rebuild_due(B.ctl)) return;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int cur = B.ctl[CTL_CUR];
+  const int hc = B.ctl[CTL_HCUR];
+  const int N = P.N;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int stride = G * blockDim.x;
+  const int first = b * blockDim.x + threadIdx.x;
+  uint32_t* hist = B.rs_hist;
+  uint32_t* dtot = B.rs_hist + (size_t)256 * G;
+  for (int s = first; s < N; s += stride) keys_one(P, B, s, cur);
+  grid.sync();
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t dpre[256];
+  for (int pass = 0; pass < 4; ++pass) {
+    const int a = pass & 1;
+    const int shift = 8 * pass;
+    rs_count_block(N, P.rs_chunk, shift, B.key[a], hist, b, G);
+    grid.sync();
+    for (int d = b; d < 256; d += G) {
+      const uint32_t tot = block_exclusive_scan(hist + (size_t)d * G, G);
+      if (threadIdx.x == 0) dtot[d] = tot;
+    }
+    grid.sync();
+    // global base of digit d = totals of smaller digits + this chunk's prefix
+    dpre[threadIdx.x] = dtot[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int d = 0; d < 256; ++d) {
+        const uint32_t c = dpre[d];
+        dpre[d] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    base[threadIdx.x] = dpre[threadIdx.x] + hist[(size_t)threadIdx.x * G + b];
+    __syncthreads();
+    rs_scatter_block(N, P.rs_chunk, shift, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1], base, b);
+    grid.sync();
+  }
+  for (int s = first; s < N; s += stride) permute_one(P, B, s, cur, hc);
+  grid.sync();
+  for (int s = first; s < N; s += stride) cells_one(P, B, s);
+  grid.sync();
+  const int nact = __hip_atomic_load(&B.ctl[CTL_NACT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int s = first; s < N; s += stride) {
+    pairlist_one(P, B, s, cur ^ 1, hc ^ 1, nact);
+    walllist_one(P, B, s, cur ^ 1, nact);
+  }
+  if (b == 0 && threadIdx.x == 0) {
+    B.ctl[CTL_CUR] = cur ^ 1;
+    B.ctl[CTL_HCUR] = hc ^ 1;
+    B.ctl[CTL_REBUILDS] += 1;
+    __hip_atomic_store(&B.ctl[CTL_REFRESH], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -932,18 +1040,34 @@ void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint
     const int a = pass & 1, b = a ^ 1;
     if (gated) {
       k_rs_count<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], hist, ctl);
-      k_rs_scan<true><<<1, 1024, 0, st>>>(256 * blocks, hist, ctl);
+      k_rs_scan<true><<<1, RS_THREADS, 0, st>>>(256 * blocks, hist, ctl);
       k_rs_scatter<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
     } else {
       k_rs_count<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], hist, ctl);
-      k_rs_scan<false><<<1, 1024, 0, st>>>(256 * blocks, hist, ctl);
+      k_rs_scan<false><<<1, RS_THREADS, 0, st>>>(256 * blocks, hist, ctl);
       k_rs_scatter<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
     }
   }
 }
 
 template <class H>
-void launch_rebuild(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
+int coop_grid_limit() {
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+  if (!prop.cooperativeLaunch) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<H>, 256, 0) != hipSuccess) return 0;
+  return per_cu * prop.multiProcessorCount;
+}
+
+template <class H>
+hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipStream_t st) {
+  if (coop) {
+    SimParams p = P;
+    Bufs<H> b = B;
+    void* args[] = {&p, &b};
+    return hipLaunchCooperativeKernel((const void*)k_rebuild_coop<H>, dim3(P.rs_blocks), dim3(256), args, 0, st);
+  }
   const int g = blocks_for(P.N, 256);
   k_keys<H><<<g, 256, 0, st>>>(P, B);
   launch_radix_sort(P.N, P.rs_blocks, P.rs_chunk, B.key[0], B.idx[0], B.key[1], B.idx[1], B.rs_hist,
@@ -952,6 +1076,7 @@ void launch_rebuild(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
   k_cells<H><<<g, 256, 0, st>>>(P, B);
   k_pairlist<H><<<g, 256, 0, st>>>(P, B);
   k_walllist<H><<<g, 256, 0, st>>>(P, B);
+  return hipGetLastError();
 }
 
 template <class H>
@@ -984,8 +1109,10 @@ template void launch_step<double, double>(const SimParams&, const Bufs<double>&,
 template void launch_step<float, float>(const SimParams&, const Bufs<float>&, hipStream_t);
 template void launch_step_record<double, double>(const SimParams&, const Bufs<double>&, const RecDev&, hipStream_t);
 template void launch_step_record<float, float>(const SimParams&, const Bufs<float>&, const RecDev&, hipStream_t);
-template void launch_rebuild<double>(const SimParams&, const Bufs<double>&, hipStream_t);
-template void launch_rebuild<float>(const SimParams&, const Bufs<float>&, hipStream_t);
+template hipError_t launch_rebuild<double>(const SimParams&, const Bufs<double>&, bool, hipStream_t);
+template hipError_t launch_rebuild<float>(const SimParams&, const Bufs<float>&, bool, hipStream_t);
+template int coop_grid_limit<double>();
+template int coop_grid_limit<float>();
 template void launch_download<double>(const SimParams&, const Bufs<double>&, double*, double*, double*, int*, hipStream_t);
 template void launch_download<float>(const SimParams&, const Bufs<float>&, double*, double*, double*, int*, hipStream_t);
 template void launch_debug<double>(const SimParams&, const Bufs<double>&, const DebugDev&, hipStream_t);

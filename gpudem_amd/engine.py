@@ -8,6 +8,7 @@ fallback: without the built library every call raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -181,6 +182,7 @@ class Context:
 
     def __init__(self, device=0):
         self.h = ctypes.c_void_p()
+        self._scenes = weakref.WeakSet()
         rc = lib().gdem_ctx_create(device, ctypes.byref(self.h))
         if rc:
             raise GdemError(rc, "gdem_ctx_create (no usable HIP device?)")
@@ -221,7 +223,10 @@ class Context:
         return s, d
 
     def close(self):
+        """Destroys every scene still open on this context, then the context."""
         if self.h:
+            for sc in list(self._scenes):
+                sc.close()
             lib().gdem_ctx_destroy(self.h)
             self.h = ctypes.c_void_p()
 
@@ -272,6 +277,7 @@ class Scene:
         self.h = ctypes.c_void_p()
         ctx.check(lib().gdem_scene_create(ctx.h, ctypes.byref(pd), ctypes.byref(md), ctypes.byref(p),
                                           ctypes.byref(self.h)), "gdem_scene_create")
+        ctx._scenes.add(self)
 
     def step(self, n=1):
         self.ctx.check(lib().gdem_scene_step(self.h, int(n)), "gdem_scene_step")
@@ -328,7 +334,8 @@ class Scene:
 
     def close(self):
         if self.h:
-            lib().gdem_scene_destroy(self.h)
+            if self.ctx.h:
+                lib().gdem_scene_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -174,7 +174,8 @@ int gdem_scene_step(gdem_scene* scene, int nsteps);
 int gdem_scene_status(gdem_scene* scene, gdem_stats* stats);
 /* Replaces copyFromDevice (ParticleSystem.h:99, SAVE_ON members). */
 int gdem_scene_download(gdem_scene* scene, gdem_particles_out* out);
-/* Replaces deallocate(ParticleSys<DeviceMemory>*) and the free_* calls. */
+/* Replaces deallocate(ParticleSys<DeviceMemory>*) and the free_* calls.
+ * Destroy every scene before the context it was created on. */
 void gdem_scene_destroy(gdem_scene* scene);
 /* Parity hooks: cell coordinates/keys of the current positions, candidate
  * lists and contact histories, all in original particle ids. */
