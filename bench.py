@@ -4,12 +4,15 @@ Workload: 1,000,000 spheres (r ~ U[0.9,1.1] x 0.5 mm) on a 4-turn helical
 chute meshed as 20,000 triangles, FP32 forces with FP64 integration, LBVH wall
 broadphase, full tangential history, dt = 2e-6 s.  W warm-up steps let the bed
 settle into contact, then K steps are timed with HIP events; the force kernel
-(the dominant one) is bracketed by its own event pair on the engine's stream.
+(the dominant one) of every n-th timed step is bracketed by its own event pair
+on the engine's stream.
 
-Prints one JSON line.  `value` is particle-steps/s over the timed K steps
-with all state resident in HBM.  `roofline` prices the force kernel's
-algorithmic bytes per launch (192 + 4 (n_nb + n_w) + 32 c bytes per active
-particle for the mixed form) against the 8 TB/s HBM peak.  `cpu_baseline`
+Prints one JSON line.  `value` is particle-steps/s over the timed K steps with
+all state resident in HBM.  `roofline` prices the force kernel's algorithmic
+bytes per launch -- 192 + 4 (n_nb + n_w) + 32 c bytes per active particle for
+the mixed form, 212 + 4 (n_nb + n_w) + 56 c for FP64, with the candidate and
+contact counts measured at the end of the window -- against the 8 TB/s HBM
+peak, and reports a plain device copy's bandwidth beside it.  `cpu_baseline`
 times the C++/OpenMP oracle (a restatement of the reference's
 cpu_dem_verlet_verlet) on the same settled state.
 """
@@ -30,27 +33,34 @@ def algorithmic_bytes(mixed, nb, nw, c):
     return base + 4.0 * (nb + nw) + per_contact * c
 
 
-def run_gpu(spec, precision, warmup, steps):
-    from gpudem_amd import Context, Scene, engine
+def make_scene(ctx, spec, precision, graph_steps, state=None):
+    from gpudem_amd import Scene, engine
     m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
-    ctx = Context(0)
-    params = dict(spec.params, precision=precision)
-    sc = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, params)
-    sc.step(warmup)
+    params = dict(spec.params, precision=precision, graph_steps=graph_steps)
+    if state is None:
+        return Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, params)
+    return Scene(ctx, state["x"], spec.r, m, k, e, spec.verts, params, v=state["v"], angv=state["angv"],
+                 active=state["active"])
+
+
+def run_gpu(ctx, spec, precision, warmup, steps, graph_steps=0, every=10, state=None):
+    sc = make_scene(ctx, spec, precision, graph_steps, state)
+    if warmup:
+        sc.step(warmup)
     ctx.sync()
     st0 = sc.status()
-    sc.set_timing(True)
-    t0 = time.perf_counter()
-    sc.step(steps)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    total_ms, force_ms, launches = sc.timing()
-    st1 = sc.status()
-    state = sc.download()
+    out = dict(stats0=st0)
+    if steps:
+        sc.set_timing(every)
+        t0 = time.perf_counter()
+        sc.step(steps)
+        ctx.sync()
+        out["wall_s"] = time.perf_counter() - t0
+        out["total_ms"], out["force_ms"], out["launches"] = sc.timing()
+    out["stats1"] = sc.status()
+    out["state"] = sc.download()
     sc.close()
-    ctx.close()
-    return dict(total_ms=total_ms, force_ms=force_ms, launches=launches, wall_s=wall, stats0=st0,
-                stats1=st1, state=state)
+    return out
 
 
 def run_cpu(spec, state, steps, warm):
@@ -68,45 +78,70 @@ def run_cpu(spec, state, steps, warm):
     return dict(value=nact * steps / dt, seconds=dt, threads=threads, n_active=nact)
 
 
+def summarize(run, steps, mixed):
+    s1 = run["stats1"]
+    nact = s1["n_active"]
+    force_avg_s = run["force_ms"] / 1e3 / max(1, run["launches"])
+    B = algorithmic_bytes(mixed, s1["mean_nb"], s1["mean_wall"], s1["mean_contacts"])
+    return dict(value=nact * steps / (run["total_ms"] / 1e3), ms_per_step=run["total_ms"] / steps,
+                force_avg_s=force_avg_s, B=B, achieved=B * nact / force_avg_s / 1e9, nact=nact,
+                rebuilds=s1["rebuilds"] - run["stats0"]["rebuilds"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=10000)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--particles", type=int, default=1000000)
+    ap.add_argument("--graph-steps", type=int, default=0,
+                    help="> 0: launch-chain rebuild captured in graphs of this many steps; "
+                         "< 0: launch-chain rebuild, direct launches")
+    ap.add_argument("--sample-every", type=int, default=10,
+                    help="bracket the force kernel of every n-th timed step with HIP events")
+    ap.add_argument("--save-state", help="write the mixed run's final state to this .npz")
+    ap.add_argument("--load-state", help="start both runs from this .npz instead of the lattice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the engine steps one "
               "scene per device and this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    from gpudem_amd import GDEM_FP64, GDEM_MIXED, scenes
+    import numpy as np
+
+    from gpudem_amd import GDEM_FP64, GDEM_MIXED, Context, scenes
     spec = scenes.helical_chute(n=args.particles)
     print(f"# workload helical_chute n={spec.n} ntri={len(spec.verts)}", file=sys.stderr, flush=True)
+    state = None
+    if args.load_state:
+        z = np.load(args.load_state)
+        state = {k: z[k] for k in ("x", "v", "angv", "active")}
+    ctx = Context(0)
+    copy_bytes = 1 << 30
+    copy_ms = ctx.copy_probe(copy_bytes, 20)
+    copy_bw = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
 
-    mixed = run_gpu(spec, GDEM_MIXED, args.warmup, args.steps)
+    mixed = run_gpu(ctx, spec, GDEM_MIXED, args.warmup, args.steps, args.graph_steps, args.sample_every, state)
+    if args.save_state:
+        np.savez(args.save_state, **mixed["state"])
+    if not args.steps:
+        print(f"# settled in {args.warmup} steps", file=sys.stderr, flush=True)
+        return
+    m = summarize(mixed, args.steps, True)
     s1 = mixed["stats1"]
-    nact = s1["n_active"]
-    value = nact * args.steps / (mixed["total_ms"] / 1e3)
-    ms_per_step = mixed["total_ms"] / args.steps
-    force_avg_s = mixed["force_ms"] / 1e3 / max(1, mixed["launches"])
-    B = algorithmic_bytes(True, s1["mean_nb"], s1["mean_wall"], s1["mean_contacts"])
-    achieved = B * nact / force_avg_s / 1e9
-    print(f"# mixed: {ms_per_step:.4f} ms/step, force kernel {force_avg_s * 1e6:.1f} us, "
-          f"B={B:.1f} B/particle-step, rebuilds {s1['rebuilds'] - mixed['stats0']['rebuilds']}",
-          file=sys.stderr, flush=True)
-
+    print(f"# mixed: {m['ms_per_step']:.4f} ms/step, force kernel {m['force_avg_s'] * 1e6:.1f} us, "
+          f"B={m['B']:.1f} B/particle-step, rebuilds {m['rebuilds']}", file=sys.stderr, flush=True)
     out = {
         "metric": "particle-steps/s at 1M spheres (% HBM roofline); batched p50/p99 scene latency",
-        "value": value,
+        "value": m["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -114,25 +149,25 @@ def main():
         "data": "synthetic (seeded helical chute and particle bed)",
         "config": {"workload": "helical_chute_1M_20k_tris_mixed", "particles": spec.n,
                    "triangles": int(len(spec.verts)), "dt": spec.params["dt"],
-                   "skin": spec.params["skin"], "cell": spec.params["cell"]},
+                   "skin": spec.params["skin"], "cell": spec.params["cell"],
+                   "rebuild": "cooperative" if args.graph_steps == 0 else "chain"},
         "list_stats": {"mean_nb": s1["mean_nb"], "mean_wall": s1["mean_wall"],
-                       "mean_contacts": s1["mean_contacts"],
-                       "rebuilds_in_window": s1["rebuilds"] - mixed["stats0"]["rebuilds"],
-                       "n_active": nact},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step<float,float>", "bytes_per_particle_step": B,
-                     "avg_launch_us": force_avg_s * 1e6},
+                       "mean_contacts": s1["mean_contacts"], "rebuilds_in_window": m["rebuilds"],
+                       "n_active": m["nact"]},
+        "roofline": {"bound": "hbm", "achieved": m["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": m["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step<float,float>", "bytes_per_particle_step": m["B"],
+                     "avg_launch_us": m["force_avg_s"] * 1e6, "sampled_launches": mixed["launches"],
+                     "copy_bw": copy_bw, "frac_of_copy": m["achieved"] / copy_bw},
     }
     if not args.no_fp64:
-        f64 = run_gpu(spec, GDEM_FP64, args.warmup, args.steps)
-        t = f64["stats1"]
-        out["fp64_value"] = t["n_active"] * args.steps / (f64["total_ms"] / 1e3)
-        fa = f64["force_ms"] / 1e3 / max(1, f64["launches"])
-        B64 = algorithmic_bytes(False, t["mean_nb"], t["mean_wall"], t["mean_contacts"])
-        out["fp64_roofline"] = {"achieved": B64 * t["n_active"] / fa / 1e9, "peak": HBM_PEAK_GBS,
-                                "unit": "GB/s", "bytes_per_particle_step": B64,
-                                "avg_launch_us": fa * 1e6}
+        f64 = run_gpu(ctx, spec, GDEM_FP64, args.warmup, args.steps, args.graph_steps, args.sample_every, state)
+        d = summarize(f64, args.steps, False)
+        out["fp64_value"] = d["value"]
+        out["fp64_roofline"] = {"achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": d["achieved"] / HBM_PEAK_GBS, "bytes_per_particle_step": d["B"],
+                                "avg_launch_us": d["force_avg_s"] * 1e6, "kernel": "k_step<double,double>"}
+    ctx.close()
     if not args.no_cpu:
         cpu = run_cpu(spec, mixed["state"], args.cpu_steps, 2)
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "particle-steps/s", "cores": cpu["threads"],

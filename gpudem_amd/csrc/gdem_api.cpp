@@ -65,13 +65,17 @@ struct gdem_scene {
   gdem::SimParams P{};
   int precision = GDEM_FP64;
   int graph_steps = 0;
-  bool coop = false;  // rebuild as one cooperative kernel (else the launch chain)
+  bool coop = false;   // rebuild as one cooperative kernel (else the launch chain)
+  bool chain = false;  // launch chain requested (graph capture or profiling)
   int64_t steps = 0;
   int rebuild_base = 0;
   gdem::MeshHost mesh;
   gdem::DevArena arena;
   gdem::TriGeo* d_tri = nullptr;
   gdem::BvhNode* d_bvh = nullptr;
+  double* d_tbox = nullptr;
+  int* d_wc_off = nullptr;
+  int* d_wc_tri = nullptr;
   int* d_ctl = nullptr;
   unsigned long long* d_acc = nullptr;
   gdem::Bufs<double> b64{};
@@ -106,10 +110,10 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
     HIPCHK(c, A.alloc(&B.mat[h], N));
     HIPCHK(c, A.alloc(&B.pid[h], N));
     HIPCHK(c, A.alloc(&B.hkey[h], N * sc->P.hist_cap));
-    HIPCHK(c, A.alloc(&B.hd[h], N * sc->P.hist_cap * 3));
+    HIPCHK(c, A.alloc(&B.hd[h], N * sc->P.hist_cap));
     HIPCHK(c, A.alloc(&B.nh[h], N));
     HIPCHK(c, A.alloc(&B.wkey[h], N * sc->P.whist_cap));
-    HIPCHK(c, A.alloc(&B.wd[h], N * sc->P.whist_cap * 3));
+    HIPCHK(c, A.alloc(&B.wd[h], N * sc->P.whist_cap));
     HIPCHK(c, A.alloc(&B.nw[h], N));
     HIPCHK(c, A.alloc(&B.key[h], N));
     HIPCHK(c, A.alloc(&B.idx[h], N));
@@ -121,17 +125,19 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
   HIPCHK(c, A.alloc(&B.nwl, N));
   HIPCHK(c, A.alloc(&B.cell_tmp, N));
   HIPCHK(c, A.alloc(&B.cell_of, N));
-  HIPCHK(c, A.alloc(&B.cell_start, ncell));
-  HIPCHK(c, A.alloc(&B.cell_end, ncell));
+  HIPCHK(c, A.alloc(&B.cell_tab, ncell));
   HIPCHK(c, A.alloc(&B.rs_hist, (size_t)256 * sc->P.rs_blocks + 256));
-  HIPCHK(c, hipMemsetAsync(B.cell_start, 0xff, ncell * sizeof(int), c->stream));
-  HIPCHK(c, hipMemsetAsync(B.cell_end, 0xff, ncell * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.cell_tab, 0xff, ncell * sizeof(int2), c->stream));
+  HIPCHK(c, hipMemsetAsync(B.cell_of, 0xff, N * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nh[0], 0, N * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nw[0], 0, N * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nnb, 0, N * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nwl, 0, N * sizeof(int), c->stream));
   B.tri = sc->d_tri;
   B.bvh = sc->d_bvh;
+  B.tbox = sc->d_tbox;
+  B.wc_off = sc->d_wc_off;
+  B.wc_tri = sc->d_wc_tri;
   B.ctl = sc->d_ctl;
   return GDEM_OK;
 }
@@ -159,16 +165,20 @@ int step_impl(gdem_scene* sc, int nsteps) {
   gdem_ctx* c = sc->ctx;
   hipStream_t st = c->stream;
   Bufs<H>& B = bufs<H>(sc);
-  if (sc->timing) {
-    // direct launches with an event pair around every force kernel
+  if (sc->timing > 0) {
+    // direct launches; an event pair brackets the force kernel of every
+    // `timing`-th step (sampling keeps the host ahead of the device)
+    const int stride = sc->timing;
+    const int nsamp = (nsteps + stride - 1) / stride;
     for (auto e : sc->ev) (void)hipEventDestroy(e);
-    sc->ev.assign((size_t)2 * nsteps, nullptr);
+    sc->ev.assign((size_t)2 * nsamp, nullptr);
     for (auto& e : sc->ev) HIPCHK(c, hipEventCreate(&e));
     HIPCHK(c, hipEventRecord(sc->t0, st));
     for (int i = 0; i < nsteps; ++i) {
-      HIPCHK(c, hipEventRecord(sc->ev[2 * i], st));
+      const bool samp = (i % stride) == 0;
+      if (samp) HIPCHK(c, hipEventRecord(sc->ev[2 * (i / stride)], st));
       launch_step<H, H>(sc->P, B, st);
-      HIPCHK(c, hipEventRecord(sc->ev[2 * i + 1], st));
+      if (samp) HIPCHK(c, hipEventRecord(sc->ev[2 * (i / stride) + 1], st));
       HIPCHK(c, launch_rebuild<H>(sc->P, B, sc->coop, st));
     }
     HIPCHK(c, hipEventRecord(sc->t1, st));
@@ -178,13 +188,13 @@ int step_impl(gdem_scene* sc, int nsteps) {
     HIPCHK(c, hipEventElapsedTime(&ms, sc->t0, sc->t1));
     sc->last_total_ms = ms;
     double fsum = 0;
-    for (int i = 0; i < nsteps; ++i) {
+    for (int i = 0; i < nsamp; ++i) {
       float m = 0;
       HIPCHK(c, hipEventElapsedTime(&m, sc->ev[2 * i], sc->ev[2 * i + 1]));
       fsum += m;
     }
     sc->last_force_ms = fsum;
-    sc->last_force_launches = nsteps;
+    sc->last_force_launches = nsamp;
     sc->steps += nsteps;
     return GDEM_OK;
   }
@@ -221,14 +231,16 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   // radix-sort / cooperative grid: one workgroup per chunk of >= 256 keys
   SimParams& P = sc->P;
   const int want = (N + 255) / 256;
-  const int limit = sc->graph_steps > 0 ? 0 : coop_grid_limit<H>();
+  const int limit = sc->chain ? 0 : coop_grid_limit<H>();
   sc->coop = limit > 0;
   P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want))
                          : std::max(1, std::min(1024, (N + 2047) / 2048));
   P.rs_chunk = ((N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
   int rc = alloc_bufs<H>(sc, B, ncell);
   if (rc) return rc;
-  std::vector<double4> pos(N), vel(N), angv(N), mat(N);
+  using H4 = typename Vec4<H>::type;
+  std::vector<double4> pos(N), vel(N), angv(N);
+  std::vector<H4> mat(N);
   std::vector<int> pid(N);
   for (int i = 0; i < N; ++i) {
     const double r = pd->r[i], m = pd->m[i];
@@ -241,14 +253,18 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
     angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., (double)i);
     const double rsq = r * r;
     const double moi = 2. / 5. * m * rsq;
-    mat[i] = make_double4(pd->k[i], pd->etaconst[i], sqrt(m), 1. / moi);
+    const double mrow[4] = {pd->k[i], pd->etaconst[i], sqrt(m), 1. / moi};
+    mat[i].x = (H)mrow[0];
+    mat[i].y = (H)mrow[1];
+    mat[i].z = (H)mrow[2];
+    mat[i].w = (H)mrow[3];
     pid[i] = i;
   }
   hipStream_t st = c->stream;
   HIPCHK(c, hipMemcpyAsync(B.pos[0], pos.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(B.vel[0], vel.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(B.angv[0], angv.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(B.mat[0], mat.data(), N * sizeof(double4), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(B.mat[0], mat.data(), N * sizeof(H4), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(B.pid[0], pid.data(), N * sizeof(int), hipMemcpyHostToDevice, st));
   int ctl[CTL_WORDS] = {0};
   ctl[CTL_REFRESH] = 1;  // the initial lists are built like main.cu:151,196 do on the host
@@ -531,6 +547,7 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
     return fail(c, GDEM_E_ARG, "params: slot capacities exceed 64 (pairs/histories) or 48 (walls)");
   sc->precision = pp->precision;
   sc->graph_steps = pp->graph_steps > 0 ? pp->graph_steps : 0;
+  sc->chain = pp->graph_steps != 0;
   // every active particle must lie in the box (the cell search assumes it)
   for (int i = 0; i < pd->n; ++i) {
     if (pd->is_active && pd->is_active[i] != 1) continue;
@@ -551,6 +568,34 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
   P.nnode = (int)sc->mesh.nodes.size();
   HIPCHK(c, sc->arena.alloc(&sc->d_tri, std::max(ntri, 1)));
   HIPCHK(c, sc->arena.alloc(&sc->d_bvh, std::max(P.nnode, 1)));
+  HIPCHK(c, sc->arena.alloc(&sc->d_tbox, (size_t)std::max(ntri, 1) * 6));
+  // static per-cell wall table over the particle grid (the mesh never moves)
+  P.wall_table = 0;
+  if (ntri > 0 && pp->wall_broadphase != 1) {
+    WallCells wc;
+    const size_t max_entries = (size_t)1 << 27;
+    const bool ok = ncell <= 4.0e8 &&
+                    build_wall_cells(sc->mesh, P.bmin, pp->cell, P.size, maxr + pp->skin, max_entries, &wc);
+    if (!ok && pp->wall_broadphase == 2)
+      return fail(c, GDEM_E_ARG, "wall table does not fit; use wall_broadphase = 0 or 1");
+    if (ok) {
+      HIPCHK(c, sc->arena.alloc(&sc->d_wc_off, wc.off.size()));
+      HIPCHK(c, sc->arena.alloc(&sc->d_wc_tri, std::max<size_t>(wc.tri.size(), 1)));
+      HIPCHK(c, hipMemcpy(sc->d_wc_off, wc.off.data(), wc.off.size() * 4, hipMemcpyHostToDevice));
+      if (!wc.tri.empty())
+        HIPCHK(c, hipMemcpy(sc->d_wc_tri, wc.tri.data(), wc.tri.size() * 4, hipMemcpyHostToDevice));
+      P.wall_table = 1;
+    }
+  }
+  if (ntri > 0) {
+    std::vector<double> tb((size_t)ntri * 6);
+    for (int t = 0; t < ntri; ++t)
+      for (int a = 0; a < 3; ++a) {
+        tb[(size_t)t * 6 + a] = sc->mesh.aabb_lo[(size_t)t * 3 + a];
+        tb[(size_t)t * 6 + 3 + a] = sc->mesh.aabb_hi[(size_t)t * 3 + a];
+      }
+    HIPCHK(c, hipMemcpy(sc->d_tbox, tb.data(), tb.size() * 8, hipMemcpyHostToDevice));
+  }
   HIPCHK(c, sc->arena.alloc(&sc->d_ctl, CTL_WORDS));
   HIPCHK(c, sc->arena.alloc(&sc->d_acc, 4));
   if (ntri > 0) {
@@ -607,7 +652,7 @@ int gdem_scene_step_record(gdem_scene* sc, gdem_contact_view* vw) {
 
 int gdem_scene_set_timing(gdem_scene* sc, int on) {
   if (!sc) return GDEM_E_ARG;
-  sc->timing = on ? 1 : 0;
+  sc->timing = on > 0 ? on : 0;
   return GDEM_OK;
 }
 
@@ -668,6 +713,31 @@ int gdem_tri_distance(gdem_ctx* c, const gdem_mesh_desc* md, int tri, int np, co
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(out, o, (size_t)np * 32, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(hit, h, (size_t)np * 4, hipMemcpyDeviceToHost));
+  return GDEM_OK;
+}
+
+int gdem_copy_probe(gdem_ctx* c, int64_t bytes, int reps, double* ms_out) {
+  if (!c || bytes < 16 || reps <= 0) return GDEM_E_ARG;
+  DevArena tmp;
+  double4 *a, *b;
+  const size_t n = (size_t)bytes / sizeof(double4);
+  HIPCHK(c, tmp.alloc(&a, n));
+  HIPCHK(c, tmp.alloc(&b, n));
+  HIPCHK(c, hipMemsetAsync(a, 0, n * sizeof(double4), c->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  launch_copy(a, b, n, c->stream);
+  HIPCHK(c, hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) launch_copy(a, b, n, c->stream);
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_out) *ms_out = ms / reps;
   return GDEM_OK;
 }
 

@@ -63,6 +63,7 @@ void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st);
 void launch_tri_probe(const TriGeo* tri, int np, const double* pts, const double* rsq, double* out,
                       int* hit, hipStream_t st);
+void launch_copy(const double4* a, double4* b, size_t n, hipStream_t st);
 void launch_fp64_probe(int n, const double* a, const double* b, double* sq, double* dv, hipStream_t st);
 
 }  // namespace gdem

@@ -164,43 +164,82 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
 }
 
 // ---------------------------------------------------------------------------
-// contact-history row of one particle, slot-major in global memory
+// contact-history row of one particle
 // ---------------------------------------------------------------------------
-template <class H>
-struct HistRow {
-  int* key;
-  H* d;
+// Global layout is slot-major: keys [cap][N] (int) and deltas [cap][N]
+// (4-vector, w unused), so lane i of a wave touches consecutive words.
+// HistLds keeps the live keys in an LDS column of the thread (CAP >= the
+// scene's capacity): they are loaded once per step with independent loads,
+// and the per-contact search costs no memory round trip.  HistScan searches
+// the keys in memory and serves capacities beyond the LDS budget.
+template <class H, int CAP>
+struct HistLds {
+  using H4 = typename Vec4<H>::type;
+  int* gkey;
+  H4* gd;
   int s, N, cap;
-  __device__ __forceinline__ int find(int k, int n) const {
+  int n;           // live slots
+  uint64_t marks;  // slots in contact this step
+  int* k;          // this thread's LDS column: k[q * blockDim.x]
+  __device__ __forceinline__ void load(int nh) {
+    n = nh;
+    marks = 0;
+#pragma unroll
+    for (int q = 0; q < CAP; ++q)
+      if (q < nh) k[q * 256] = gkey[(size_t)q * N + s];
+  }
+  __device__ __forceinline__ int find(int key) const {
     for (int q = 0; q < n; ++q)
-      if (key[(size_t)q * N + s] == k) return q;
+      if (k[q * 256] == key) return q;
     return -1;
   }
-  __device__ __forceinline__ H get(int q, int c) const { return d[((size_t)q * 3 + c) * N + s]; }
-  __device__ __forceinline__ void put(int q, int c, H v) const { d[((size_t)q * 3 + c) * N + s] = v; }
+  __device__ __forceinline__ void remember(int key) { k[n * 256] = key; }
+};
+
+template <class H>
+struct HistScan {
+  using H4 = typename Vec4<H>::type;
+  int* gkey;
+  H4* gd;
+  int s, N, cap;
+  int n;
+  uint64_t marks;
+  __device__ __forceinline__ void load(int nh) {
+    n = nh;
+    marks = 0;
+  }
+  __device__ __forceinline__ int find(int key) const {
+    for (int q = 0; q < n; ++q)
+      if (gkey[(size_t)q * N + s] == key) return q;
+    return -1;
+  }
+  __device__ __forceinline__ void remember(int) {}
 };
 
 // Tangential spring with Coulomb cap and history (d_calc_tangential_force,
 // device_dem.cu:622-740; the wall twin :403-499 is identical with the wall
-// history).  T is the force type; FP64 keeps the reference's operations.
-template <class T, class H>
-__device__ __forceinline__ V3<T> tangential(const HistRow<H>& row, int& nh, uint64_t& marks,
-                                            uint32_t& err, uint32_t ovf_bit, int key, V3<T> fn,
+// history).  The slot of `key` is found or appended (d_calc_tangential_force
+// :630-653); T is the force type and FP64 keeps the reference's operations.
+template <class T, class Row>
+__device__ __forceinline__ V3<T> tangential(Row& row, uint32_t& err, uint32_t ovf_bit, int key, V3<T> fn,
                                             V3<T> vt, V3<T> n, T eta, T k, T mu, T dt) {
-  int slot = row.find(key, nh);
+  using H4 = typename Row::H4;
+  int slot = row.find(key);
   V3<T> old{T(0), T(0), T(0)};
   if (slot < 0) {
-    if (nh < row.cap) {
-      slot = nh;
-      row.key[(size_t)slot * row.N + row.s] = key;
-      ++nh;
+    if (row.n < row.cap) {
+      slot = row.n;
+      row.gkey[(size_t)slot * row.N + row.s] = key;
+      row.remember(key);
+      ++row.n;
     } else {
       err |= ovf_bit;
     }
   } else {
-    old = V3<T>{(T)row.get(slot, 0), (T)row.get(slot, 1), (T)row.get(slot, 2)};
+    const H4 d = row.gd[(size_t)slot * row.N + row.s];
+    old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
   }
-  if (slot >= 0) marks |= (uint64_t)1 << slot;
+  if (slot >= 0) row.marks |= (uint64_t)1 << slot;
   V3<T> dn{old.x + vt.x * dt, old.y + vt.y * dt, old.z + vt.z * dt};
   const T dd = dot(dn, n);
   dn = vsub(dn, scal(dd, n));
@@ -222,9 +261,12 @@ __device__ __forceinline__ V3<T> tangential(const HistRow<H>& row, int& nh, uint
     ft.z -= eta * vt.z;
   }
   if (slot >= 0) {
-    row.put(slot, 0, (H)dn.x);
-    row.put(slot, 1, (H)dn.y);
-    row.put(slot, 2, (H)dn.z);
+    H4 o;
+    o.x = (decltype(o.x))dn.x;
+    o.y = (decltype(o.y))dn.y;
+    o.z = (decltype(o.z))dn.z;
+    o.w = 0;
+    row.gd[(size_t)slot * row.N + row.s] = o;
   }
   return ft;
 }
@@ -249,12 +291,50 @@ struct Rec {
   double* delta;
 };
 
+// The hardware hands workgroup b to XCD b % 8.  Giving each XCD a contiguous
+// run of tiles keeps Morton-adjacent particles -- the neighbours a tile
+// gathers -- inside one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int X = 8;
+  if (nb < X) return b;
+  const int per = nb / X, rem = nb % X;
+  const int x = b % X, i = b / X;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // ---------------------------------------------------------------------------
 // fused contact forces + integration for one step
 // ---------------------------------------------------------------------------
-template <class T, class H, bool REC>
+// PRow/WRow: history row types for pairs and walls.  Pair candidates are
+// fetched four at a time (indices, then positions) so their latencies
+// overlap; contacts are still evaluated in list order.
+template <int PC, int WC>
+struct StepLds {
+  int pkey[PC > 0 ? PC : 1][256];
+  int wkey[WC > 0 ? WC : 1][256];
+  int ve[8][256];
+};
+
+template <class Row>
+__device__ __forceinline__ void bind_lds(Row&, int*) {}
+template <class H, int CAP>
+__device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col) { r.k = col; }
+
+template <class Row>
+struct RowCap {
+  static constexpr int value = 0;
+};
+template <class H, int CAP>
+struct RowCap<HistLds<H, CAP>> {
+  static constexpr int value = CAP;
+};
+
+template <class T, class H, bool REC, class PRow, class WRow>
 __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  using H4 = typename Vec4<H>::type;
+  __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
+  const int tx = threadIdx.x;
+  const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int cur = __builtin_amdgcn_readfirstlane(B.ctl[CTL_CUR]);
   const int hc = __builtin_amdgcn_readfirstlane(B.ctl[CTL_HCUR]);
   const int N = P.N;
@@ -272,9 +352,9 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
       B.vel[cur ^ 1][s] = v;
       B.angv[cur ^ 1][s] = w;
     } else {
-      const double4 mt = B.mat[hc][s];
+      const H4 mt = B.mat[hc][s];
       const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
-      const double invmoi = mt.w;
+      const double invmoi = (double)mt.w;
       const double ri = p.w, invm = v.w;
       const T riT = (T)ri;
       const T mu = (T)P.mu, dtT = (T)P.dt;
@@ -283,82 +363,105 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
       V3<T> f{T(0), T(0), T(0)}, mom{T(0), T(0), T(0)};
       int nrec = 0;
       const int self_id = (int)w.w;
+      const int nnb = B.nnb[s];
+      const int nwl = B.nwl[s];
 
       // ---- particle pairs -------------------------------------------------
       {
-        const HistRow<H> row{B.hkey[hc], B.hd[hc], s, N, P.hist_cap};
-        int nh = B.nh[hc][s];
-        uint64_t marks = 0;
-        const int nnb = B.nnb[s];
-        for (int q = 0; q < nnb; ++q) {
-          const int j = B.nbr[(size_t)q * N + s];
-          const double4 pj = pin[j];
-          if (pj.w < 0.) continue;
-          const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-          const double distsq = dx * dx + dy * dy + dz * dz;
-          const double R = ri + pj.w;
-          if (!(distsq < R * R)) continue;
-          const double dist = sqrt(distsq);
-          const double delMagD = R - dist;
-          V3<T> n;
-          if constexpr (sizeof(T) == 8) {
-            n = V3<T>{dx / dist, dy / dist, dz / dist};
-          } else {
-            const T inv = T(1) / (T)dist;
-            n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
-          }
-          const T delMag = (T)delMagD;
-          const double4 vj = vin[j];
-          const double4 wj = win[j];
-          const V3<T> del = scal(-delMag, n);
-          const V3<T> vr{vi.x - (T)vj.x, vi.y - (T)vj.y, vi.z - (T)vj.z};
-          const T vrn = dot(vr, n);
-          const V3<T> vn = scal(vrn, n);
-          const T meff = T(1) / ((T)invm + (T)vj.w);
-          const T eta = etac * sqrt(meff);
-          V3<T> fn;
-          fn.x = -k * del.x - eta * vn.x;
-          fn.y = -k * del.y - eta * vn.y;
-          fn.z = -k * del.z - eta * vn.z;
-          const T rj = (T)pj.w;
-          V3<T> vrot{riT * wi.x + rj * (T)wj.x, riT * wi.y + rj * (T)wj.y, riT * wi.z + rj * (T)wj.z};
-          vrot = cross(vrot, n);
-          const V3<T> vt = vadd(vsub(vr, vn), vrot);
-          f.x += fn.x;
-          f.y += fn.y;
-          f.z += fn.z;
-          const int key = (int)wj.w;  // original id of j
-          const V3<T> ft = tangential<T, H>(row, nh, marks, err, GDEM_ERR_HIST_OVERFLOW_BIT, key, fn,
-                                            vt, n, eta, k, mu, dtT);
-          f.x += ft.x;
-          f.y += ft.y;
-          f.z += ft.z;
-          const V3<T> mm = scal(riT, cross(n, ft));
-          mom.x += mm.x;
-          mom.y += mm.y;
-          mom.z += mm.z;
-          if constexpr (REC) {
-            if (nrec < rec.cap) {
-              const size_t o = (size_t)self_id * rec.cap + nrec;
-              rec.key[o] = key;
-              rec.kind[o] = 0;
-              rec.fn[o * 3] = fn.x; rec.fn[o * 3 + 1] = fn.y; rec.fn[o * 3 + 2] = fn.z;
-              rec.ft[o * 3] = ft.x; rec.ft[o * 3 + 1] = ft.y; rec.ft[o * 3 + 2] = ft.z;
-              rec.delta[o] = delMagD;
-              ++nrec;
+        PRow row;
+        bind_lds(row, &lds.pkey[0][tx]);
+        row.gkey = B.hkey[hc];
+        row.gd = B.hd[hc];
+        row.s = s;
+        row.N = N;
+        row.cap = P.hist_cap;
+        row.load(B.nh[hc][s]);
+        for (int q0 = 0; q0 < nnb; q0 += 4) {
+          // fetch four candidates (indices, then positions) before using any
+          int j0 = B.nbr[(size_t)q0 * N + s];
+          int j1 = (q0 + 1 < nnb) ? B.nbr[(size_t)(q0 + 1) * N + s] : -1;
+          int j2 = (q0 + 2 < nnb) ? B.nbr[(size_t)(q0 + 2) * N + s] : -1;
+          int j3 = (q0 + 3 < nnb) ? B.nbr[(size_t)(q0 + 3) * N + s] : -1;
+          double4 p0 = pin[j0];
+          double4 p1 = j1 >= 0 ? pin[j1] : make_double4(0., 0., 0., -1.);
+          double4 p2 = j2 >= 0 ? pin[j2] : make_double4(0., 0., 0., -1.);
+          double4 p3 = j3 >= 0 ? pin[j3] : make_double4(0., 0., 0., -1.);
+          for (int u = 0; u < 4; ++u) {
+            const int j = j0;
+            const double4 pj = p0;
+            j0 = j1; j1 = j2; j2 = j3; j3 = -1;
+            p0 = p1; p1 = p2; p2 = p3;
+            if (j < 0) break;
+            if (pj.w < 0.) continue;
+            const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+            const double distsq = dx * dx + dy * dy + dz * dz;
+            const double R = ri + pj.w;
+            if (!(distsq < R * R)) continue;
+            const double dist = sqrt(distsq);
+            const double delMagD = R - dist;
+            V3<T> n;
+            if constexpr (sizeof(T) == 8) {
+              n = V3<T>{dx / dist, dy / dist, dz / dist};
+            } else {
+              const T inv = T(1) / (T)dist;
+              n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
+            }
+            const T delMag = (T)delMagD;
+            const double4 vj = vin[j];
+            const double4 wj = win[j];
+            const V3<T> del = scal(-delMag, n);
+            const V3<T> vr{vi.x - (T)vj.x, vi.y - (T)vj.y, vi.z - (T)vj.z};
+            const T vrn = dot(vr, n);
+            const V3<T> vn = scal(vrn, n);
+            const T meff = T(1) / ((T)invm + (T)vj.w);
+            const T eta = etac * sqrt(meff);
+            V3<T> fn;
+            fn.x = -k * del.x - eta * vn.x;
+            fn.y = -k * del.y - eta * vn.y;
+            fn.z = -k * del.z - eta * vn.z;
+            const T rj = (T)pj.w;
+            V3<T> vrot{riT * wi.x + rj * (T)wj.x, riT * wi.y + rj * (T)wj.y, riT * wi.z + rj * (T)wj.z};
+            vrot = cross(vrot, n);
+            const V3<T> vt = vadd(vsub(vr, vn), vrot);
+            f.x += fn.x;
+            f.y += fn.y;
+            f.z += fn.z;
+            const int key = (int)wj.w;  // original id of j
+            const V3<T> ft = tangential<T>(row, err, GDEM_ERR_HIST_OVERFLOW_BIT, key, fn, vt, n, eta, k, mu, dtT);
+            f.x += ft.x;
+            f.y += ft.y;
+            f.z += ft.z;
+            const V3<T> mm = scal(riT, cross(n, ft));
+            mom.x += mm.x;
+            mom.y += mm.y;
+            mom.z += mm.z;
+            if constexpr (REC) {
+              if (nrec < rec.cap) {
+                const size_t o = (size_t)self_id * rec.cap + nrec;
+                rec.key[o] = key;
+                rec.kind[o] = 0;
+                rec.fn[o * 3] = fn.x; rec.fn[o * 3 + 1] = fn.y; rec.fn[o * 3 + 2] = fn.z;
+                rec.ft[o * 3] = ft.x; rec.ft[o * 3 + 1] = ft.y; rec.ft[o * 3 + 2] = ft.z;
+                rec.delta[o] = delMagD;
+                ++nrec;
+              }
             }
           }
         }
-        B.nh[hc][s] = truncate_history(nh, marks);
+        B.nh[hc][s] = truncate_history(row.n, row.marks);
       }
 
       // ---- walls ------------------------------------------------------------
       {
-        const HistRow<H> row{B.wkey[hc], B.wd[hc], s, N, P.whist_cap};
-        int nw = B.nw[hc][s];
-        uint64_t marks = 0;
-        const int nwl = B.nwl[s];
-        int ve[8];
+        WRow row;
+        bind_lds(row, &lds.wkey[0][tx]);
+        row.gkey = B.wkey[hc];
+        row.gd = B.wd[hc];
+        row.s = s;
+        row.N = N;
+        row.cap = P.whist_cap;
+        row.load(B.nw[hc][s]);
+        int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
         int nve = 0;
         const double rsq = ri * ri;
         const T eta = etac * sqrtm;
@@ -368,17 +471,15 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
           if (!(h.dist < ri)) continue;
           if (h.hit != -1) {
             bool dup = false;
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-              if (u < nve && ve[u] == h.hit) dup = true;
+            const int seen = nve < 8 ? nve : 8;
+            for (int u = 0; u < seen; ++u)
+              if (ve[u * 256] == h.hit) {
+                dup = true;
+                break;
+              }
             if (dup) continue;
-            if (nve < 8) {
-#pragma unroll
-              for (int u = 0; u < 8; ++u)
-                if (u == nve) ve[u] = h.hit;
-            } else {
-              err |= GDEM_ERR_VE_OVERFLOW_BIT;
-            }
+            if (nve < 8) ve[nve * 256] = h.hit;
+            else err |= GDEM_ERR_VE_OVERFLOW_BIT;
             ++nve;
           }
           const double delMagD = ri - h.dist;
@@ -397,8 +498,7 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
           f.x += fn.x;
           f.y += fn.y;
           f.z += fn.z;
-          const V3<T> ft = tangential<T, H>(row, nw, marks, err, GDEM_ERR_WHIST_OVERFLOW_BIT, t, fn,
-                                            vt, n, eta, k, mu, dtT);
+          const V3<T> ft = tangential<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, t, fn, vt, n, eta, k, mu, dtT);
           f.x += ft.x;
           f.y += ft.y;
           f.z += ft.z;
@@ -418,7 +518,7 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
             }
           }
         }
-        B.nw[hc][s] = truncate_history(nw, marks);
+        B.nw[hc][s] = truncate_history(row.n, row.marks);
       }
       if constexpr (REC) rec.count[self_id] = nrec;
 
@@ -475,8 +575,12 @@ __device__ __forceinline__ bool rebuild_due(const int* ctl) {
 }
 
 // Cell coordinates (d_calcCellId, BoundingBox.cu:1089-1104) and sort keys.
+// Also clears the table entry of the cell this particle occupied at the last
+// rebuild, so the new table holds exactly the occupied cells.
 template <class H>
 __device__ __forceinline__ void keys_one(const SimParams& P, const Bufs<H>& B, int s, int cur) {
+  const int old = B.cell_of[s];
+  if (old >= 0) B.cell_tab[old] = make_int2(-1, -1);
   const double4 p = B.pos[cur][s];
   uint32_t key = 0xffffffffu;
   int cid = -1;
@@ -606,20 +710,18 @@ __device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B
   B.nh[hc ^ 1][s] = nh;
   for (int q = 0; q < nh; ++q) {
     B.hkey[hc ^ 1][(size_t)q * N + s] = B.hkey[hc][(size_t)q * N + src];
-    for (int c = 0; c < 3; ++c)
-      B.hd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.hd[hc][((size_t)q * 3 + c) * N + src];
+    B.hd[hc ^ 1][(size_t)q * N + s] = B.hd[hc][(size_t)q * N + src];
   }
   const int nw = B.nw[hc][src];
   B.nw[hc ^ 1][s] = nw;
   for (int q = 0; q < nw; ++q) {
     B.wkey[hc ^ 1][(size_t)q * N + s] = B.wkey[hc][(size_t)q * N + src];
-    for (int c = 0; c < 3; ++c)
-      B.wd[hc ^ 1][((size_t)q * 3 + c) * N + s] = B.wd[hc][((size_t)q * 3 + c) * N + src];
+    B.wd[hc ^ 1][(size_t)q * N + s] = B.wd[hc][(size_t)q * N + src];
   }
 }
 
-// Cell table from the sorted keys: start/end of each occupied cell.  Entries
-// of empty cells are stale; lookups validate them through cell_of.
+// Cell table from the sorted keys: first/end of each occupied cell (the
+// entries of cells occupied at the last rebuild were cleared by keys_one).
 template <class H>
 __device__ __forceinline__ void cells_one(const SimParams& P, const Bufs<H>& B, int s) {
   const int N = P.N;
@@ -630,9 +732,9 @@ __device__ __forceinline__ void cells_one(const SimParams& P, const Bufs<H>& B, 
     return;
   }
   const int c = B.cell_of[s];
-  if (s == 0 || key[s - 1] != k) B.cell_start[c] = s;
+  if (s == 0 || key[s - 1] != k) B.cell_tab[c].x = s;
   const bool last = (s == N - 1) || key[s + 1] != k;
-  if (last) B.cell_end[c] = s + 1;
+  if (last) B.cell_tab[c].y = s + 1;
   if ((s == N - 1) || key[s + 1] == 0xffffffffu) B.ctl[CTL_NACT] = s + 1;
 }
 
@@ -673,9 +775,9 @@ __device__ __forceinline__ void pairlist_one(const SimParams& P, const Bufs<H>& 
         const int X = cx + dx;
         if (X < 0 || X >= sx) continue;
         const int cc = (sy * Z + Y) * sx + X;
-        const int st = B.cell_start[cc];
-        if (st < 0 || st >= nact || B.cell_of[st] != cc) continue;
-        const int en = B.cell_end[cc];
+        const int2 ce = B.cell_tab[cc];
+        if (ce.x < 0) continue;
+        const int st = ce.x, en = ce.y;
         for (int q = st; q < en; ++q) {
           if (q == s) continue;
           const double4 pj = pos[q];
@@ -726,40 +828,63 @@ __device__ __forceinline__ void walllist_one(const SimParams& P, const Bufs<H>& 
   int list[WL_MAX];
   int n = 0;
   bool ovf = false;
-  int stack[64];
-  int sp = 0;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    const BvhNode* nd = B.bvh + stack[--sp];
-    const int nchild = (nd->child[0] == nd->child[1]) ? 1 : 2;
-    for (int c = 0; c < nchild; ++c) {
-      double ex = 0.0, ey = 0.0, ez = 0.0;
-      if (p.x < nd->lo[c][0]) ex = nd->lo[c][0] - p.x;
-      else if (p.x > nd->hi[c][0]) ex = p.x - nd->hi[c][0];
-      if (p.y < nd->lo[c][1]) ey = nd->lo[c][1] - p.y;
-      else if (p.y > nd->hi[c][1]) ey = p.y - nd->hi[c][1];
-      if (p.z < nd->lo[c][2]) ez = nd->lo[c][2] - p.z;
-      else if (p.z > nd->hi[c][2]) ez = p.z - nd->hi[c][2];
-      if (!((ex * ex + ey * ey + ez * ez) <= Rsq)) continue;
-      const int ch = nd->child[c];
-      if (ch >= 0) {
-        if (sp < 64) stack[sp++] = ch;
-        else ovf = true;
-      } else {
-        const int t = ~ch;
-        const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, Rsq);
-        if (h.dist < p.w + P.skin) {
-          if (n >= cap) {
-            ovf = true;
-            continue;
+  // sphereAABBOverlapNeighbor (BVH.cu:542-574) then the exact test
+  auto consider = [&](int t, const double* lo, const double* hi) {
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    if (p.x < lo[0]) ex = lo[0] - p.x;
+    else if (p.x > hi[0]) ex = p.x - hi[0];
+    if (p.y < lo[1]) ey = lo[1] - p.y;
+    else if (p.y > hi[1]) ey = p.y - hi[1];
+    if (p.z < lo[2]) ez = lo[2] - p.z;
+    else if (p.z > hi[2]) ez = p.z - hi[2];
+    return (ex * ex + ey * ey + ez * ez) <= Rsq;
+  };
+  if (P.wall_table) {
+    // static per-cell table: ascending triangle ids already
+    const int c = B.cell_of[s];
+    const int b1 = B.wc_off[c + 1];
+    for (int q = B.wc_off[c]; q < b1; ++q) {
+      const int t = B.wc_tri[q];
+      const double* bx = B.tbox + (size_t)t * 6;
+      if (!consider(t, bx, bx + 3)) continue;
+      const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, Rsq);
+      if (h.dist < p.w + P.skin) {
+        if (n >= cap) {
+          ovf = true;
+          continue;
+        }
+        list[n++] = t;
+      }
+    }
+  } else {
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      const BvhNode* nd = B.bvh + stack[--sp];
+      const int nchild = (nd->child[0] == nd->child[1]) ? 1 : 2;
+      for (int c = 0; c < nchild; ++c) {
+        if (!consider(-1, nd->lo[c], nd->hi[c])) continue;
+        const int ch = nd->child[c];
+        if (ch >= 0) {
+          if (sp < 64) stack[sp++] = ch;
+          else ovf = true;
+        } else {
+          const int t = ~ch;
+          const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, Rsq);
+          if (h.dist < p.w + P.skin) {
+            if (n >= cap) {
+              ovf = true;
+              continue;
+            }
+            int at = n;
+            while (at > 0 && list[at - 1] > t) {
+              list[at] = list[at - 1];
+              --at;
+            }
+            list[at] = t;
+            ++n;
           }
-          int at = n;
-          while (at > 0 && list[at - 1] > t) {
-            list[at] = list[at - 1];
-            --at;
-          }
-          list[at] = t;
-          ++n;
         }
       }
     }
@@ -957,15 +1082,19 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
   D.nhist[id] = nh;
   for (int q = 0; q < nh; ++q) {
     D.hist_key[(size_t)id * P.hist_cap + q] = B.hkey[hc][(size_t)q * N + s];
-    for (int c = 0; c < 3; ++c)
-      D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + c] = (double)B.hd[hc][((size_t)q * 3 + c) * N + s];
+    const auto d = B.hd[hc][(size_t)q * N + s];
+    D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + 0] = (double)d.x;
+    D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + 1] = (double)d.y;
+    D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + 2] = (double)d.z;
   }
   const int nw2 = B.nw[hc][s];
   D.nwhist[id] = nw2;
   for (int q = 0; q < nw2; ++q) {
     D.whist_key[(size_t)id * P.whist_cap + q] = B.wkey[hc][(size_t)q * N + s];
-    for (int c = 0; c < 3; ++c)
-      D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + c] = (double)B.wd[hc][((size_t)q * 3 + c) * N + s];
+    const auto d = B.wd[hc][(size_t)q * N + s];
+    D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + 0] = (double)d.x;
+    D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + 1] = (double)d.y;
+    D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + 2] = (double)d.z;
   }
 }
 
@@ -1008,6 +1137,17 @@ __global__ void k_tri_probe(const TriGeo* tri, int np, const double* pts, const 
   hit[i] = h.hit;
 }
 
+// plain streaming copy (calibrates the HBM byte counters and the achievable
+// bandwidth): 32 B per lane, grid-stride
+__global__ __launch_bounds__(256) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+void launch_copy(const double4* a, double4* b, size_t n, hipStream_t st) {
+  k_copy<<<4096, 256, 0, st>>>(a, b, n);
+}
+
 __global__ void k_fp64_probe(int n, const double* a, const double* b, double* sq, double* dv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1020,16 +1160,24 @@ __global__ void k_fp64_probe(int n, const double* a, const double* b, double* sq
 // ---------------------------------------------------------------------------
 static inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
 
+// LDS-held history keys when the capacities fit (pairs <= 16, walls <= 8).
+constexpr int PCAP_REG = 16;
+constexpr int WCAP_REG = 8;
+
 template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
   Rec<T> none{};
-  k_step<T, H, false><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, none);
+  const int g = blocks_for(P.N, 256);
+  if (P.hist_cap <= PCAP_REG && P.whist_cap <= WCAP_REG)
+    k_step<T, H, false, HistLds<H, PCAP_REG>, HistLds<H, WCAP_REG>><<<g, 256, 0, st>>>(P, B, none);
+  else
+    k_step<T, H, false, HistScan<H>, HistScan<H>><<<g, 256, 0, st>>>(P, B, none);
 }
 
 template <class T, class H>
 void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st) {
   Rec<T> rec{r.cap, r.count, r.key, r.kind, r.fn, r.ft, r.delta};
-  k_step<T, H, true><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
+  k_step<T, H, true, HistScan<H>, HistScan<H>><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
 }
 
 void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1,

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -201,6 +202,95 @@ bool build_mesh(const double* verts, int ntri, const int* tris_per_file, int nfi
     }
   }
   // A leaf's box must be exactly its triangle's box (the candidate test).
+  return true;
+}
+
+namespace {
+
+// Squared distance from p to the closest point of triangle (a, b, c), by
+// Voronoi-region classification (Ericson, Real-Time Collision Detection,
+// 5.1.5).  Used only as a conservative bound.
+double point_tri_dist2(const double* p, const double* a, const double* b, const double* c) {
+  double ab[3], ac[3], ap[3];
+  for (int i = 0; i < 3; ++i) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ap[i] = p[i] - a[i]; }
+  auto dot = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  auto d2 = [&](const double* q) {
+    const double e[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+    return dot(e, e);
+  };
+  const double d1 = dot(ab, ap), d2v = dot(ac, ap);
+  if (d1 <= 0 && d2v <= 0) return d2(a);
+  double bp[3];
+  for (int i = 0; i < 3; ++i) bp[i] = p[i] - b[i];
+  const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0 && d4 <= d3) return d2(b);
+  const double vc = d1 * d4 - d3 * d2v;
+  if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+    const double v = d1 / (d1 - d3);
+    const double q[3] = {a[0] + v * ab[0], a[1] + v * ab[1], a[2] + v * ab[2]};
+    return d2(q);
+  }
+  double cp[3];
+  for (int i = 0; i < 3; ++i) cp[i] = p[i] - c[i];
+  const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0 && d5 <= d6) return d2(c);
+  const double vb = d5 * d2v - d1 * d6;
+  if (vb <= 0 && d2v >= 0 && d6 <= 0) {
+    const double w = d2v / (d2v - d6);
+    const double q[3] = {a[0] + w * ac[0], a[1] + w * ac[1], a[2] + w * ac[2]};
+    return d2(q);
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const double q[3] = {b[0] + w * (c[0] - b[0]), b[1] + w * (c[1] - b[1]), b[2] + w * (c[2] - b[2])};
+    return d2(q);
+  }
+  const double den = 1.0 / (va + vb + vc);
+  const double v = vb * den, w = vc * den;
+  const double q[3] = {a[0] + ab[0] * v + ac[0] * w, a[1] + ab[1] * v + ac[1] * w, a[2] + ab[2] * v + ac[2] * w};
+  return d2(q);
+}
+
+}  // namespace
+
+bool build_wall_cells(const MeshHost& m, const double bmin[3], double cell, const int size[3],
+                      double reach, size_t max_entries, WallCells* out) {
+  const int ntri = (int)m.tri.size();
+  const size_t ncell = (size_t)size[0] * size[1] * size[2];
+  const double half_diag = 0.5 * cell * std::sqrt(3.0);
+  // conservative radius with a relative margin for rounding
+  const double rad = (half_diag + reach) * (1.0 + 1e-9) + 1e-12;
+  const double rad2 = rad * rad;
+  const double inv = 1.0 / cell;
+  std::vector<std::pair<int, int>> pairs;  // (cell, triangle), triangles visited in ascending id
+  for (int t = 0; t < ntri; ++t) {
+    const TriGeo& g = m.tri[t];
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+      const double l = m.aabb_lo[(size_t)t * 3 + a] - rad, h = m.aabb_hi[(size_t)t * 3 + a] + rad;
+      lo[a] = std::max(0, (int)std::floor((l - bmin[a]) * inv) + 1);
+      hi[a] = std::min(size[a] - 1, (int)std::floor((h - bmin[a]) * inv) + 1);
+    }
+    for (int cz = lo[2]; cz <= hi[2]; ++cz)
+      for (int cy = lo[1]; cy <= hi[1]; ++cy)
+        for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+          // cell cx spans [bmin + (cx-1) cell, bmin + cx cell)
+          const double c[3] = {bmin[0] + (cx - 0.5) * cell, bmin[1] + (cy - 0.5) * cell,
+                               bmin[2] + (cz - 0.5) * cell};
+          const double pd = g.n[0] * c[0] + g.n[1] * c[1] + g.n[2] * c[2] + g.d;
+          if (std::fabs(pd) > rad) continue;
+          if (point_tri_dist2(c, g.p0, g.p1, g.p2) > rad2) continue;
+          pairs.emplace_back((size_t)(size[1] * cz + cy) * size[0] + cx, t);
+          if (pairs.size() > max_entries) return false;
+        }
+  }
+  out->off.assign(ncell + 1, 0);
+  for (auto& pr : pairs) out->off[(size_t)pr.first + 1]++;
+  for (size_t c = 0; c < ncell; ++c) out->off[c + 1] += out->off[c];
+  out->tri.assign(pairs.size(), 0);
+  std::vector<int> fill(out->off.begin(), out->off.end() - 1);
+  for (auto& pr : pairs) out->tri[fill[pr.first]++] = pr.second;  // stays ascending per cell
   return true;
 }
 

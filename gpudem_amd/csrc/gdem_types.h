@@ -46,6 +46,7 @@ struct SimParams {
   int nb_cap, wall_cap, hist_cap, whist_cap;
   int ntri, nnode;
   int rs_blocks, rs_chunk;  // radix sort grid and per-block chunk
+  int wall_table;           // 1: wall candidates from the static per-cell table, 0: BVH traversal
 };
 
 // Control words shared by all kernels of a scene (device memory).
@@ -62,20 +63,33 @@ enum Ctl : int {
   CTL_WORDS = 16
 };
 
+// 4-wide vector of the history/material precision.
+template <class H>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
 // All device buffers of a scene.  Passed by value; which half of a [2]
 // array is current is read from the control words at kernel start.
 template <class H>
 struct Bufs {
+  using H4 = typename Vec4<H>::type;
   double4* pos[2];   // x, y, z, r   (r < 0 marks an inactive particle)
   double4* vel[2];   // vx, vy, vz, 1/m
   double4* angv[2];  // wx, wy, wz, unused
-  double4* mat[2];   // k, etaconst, sqrt(m), 1/I
+  H4* mat[2];        // k, etaconst, sqrt(m), 1/I (FP64, or FP32 in the mixed form)
   int* pid[2];       // original particle id
   int* hkey[2];      // pair history keys, slot-major [hist_cap][N]
-  H* hd[2];          // pair history deltas, slot-major [hist_cap][3][N]
+  H4* hd[2];         // pair history deltas (x, y, z, unused), slot-major [hist_cap][N]
   int* nh[2];
   int* wkey[2];      // wall history keys [whist_cap][N]
-  H* wd[2];          // [whist_cap][3][N]
+  H4* wd[2];         // [whist_cap][N]
   int* nw[2];
   double4* ref;      // positions at the last list build
   int* nbr;          // pair candidates (internal index), slot-major [nb_cap][N]
@@ -86,11 +100,13 @@ struct Bufs {
   int* idx[2];
   int* cell_tmp;     // cell id per particle before the permutation
   int* cell_of;      // cell id per particle after it
-  int* cell_start;   // [ncell], validated through cell_of
-  int* cell_end;
+  int2* cell_tab;    // [ncell] (first, end) of each occupied cell, (-1, -1) otherwise
   uint32_t* rs_hist; // [256 * rs_blocks]
   const TriGeo* tri;
   const BvhNode* bvh;
+  const double* tbox;   // per-triangle box [ntri][6]: lo xyz, hi xyz
+  const int* wc_off;    // per-cell wall table [ncell + 1]
+  const int* wc_tri;
   int* ctl;
 };
 

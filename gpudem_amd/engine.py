@@ -25,7 +25,8 @@ SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error"
            "gdem_bbox_rule", "gdem_params_default", "gdem_scene_create", "gdem_scene_step",
            "gdem_scene_status", "gdem_scene_download", "gdem_scene_destroy", "gdem_scene_debug",
            "gdem_scene_step_record", "gdem_scene_timing", "gdem_scene_set_timing",
-           "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute"]
+           "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
+           "gdem_copy_probe"]
 
 
 class GdemError(RuntimeError):
@@ -52,7 +53,7 @@ class Params(ctypes.Structure):
                 ("skin", ctypes.c_double), ("refresh", ctypes.c_double),
                 ("precision", ctypes.c_int), ("nb_cap", ctypes.c_int), ("wall_cap", ctypes.c_int),
                 ("hist_cap", ctypes.c_int), ("whist_cap", ctypes.c_int),
-                ("graph_steps", ctypes.c_int)]
+                ("graph_steps", ctypes.c_int), ("wall_broadphase", ctypes.c_int)]
 
 
 class ParticlesOut(ctypes.Structure):
@@ -113,6 +114,7 @@ def lib():
         L.gdem_tri_distance.argtypes = [vp, P(MeshDesc), ctypes.c_int, ctypes.c_int, vp, vp, vp, vp]
         L.gdem_fp64_probe.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.gdem_mesh_precompute.argtypes = [P(MeshDesc), vp, vp, vp]
+        L.gdem_copy_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -214,6 +216,12 @@ class Context:
                                            _ptr(out), _ptr(hit)), "tri_distance")
         return out, hit
 
+    def copy_probe(self, nbytes, reps=20):
+        """Average duration (ms) of a plain device copy of nbytes."""
+        ms = ctypes.c_double()
+        self.check(lib().gdem_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(ms)), "copy_probe")
+        return ms.value
+
     def fp64_probe(self, a, b):
         a = _f64(a)
         b = _f64(b)
@@ -272,6 +280,7 @@ class Scene:
         p.hist_cap = params.get("hist_cap", 0)
         p.whist_cap = params.get("whist_cap", 0)
         p.graph_steps = params.get("graph_steps", 0)
+        p.wall_broadphase = params.get("wall_broadphase", 0)
         self.caps = dict(nb=p.nb_cap or 48, wall=p.wall_cap or 24, hist=p.hist_cap or 16,
                          whist=p.whist_cap or 8)
         self.h = ctypes.c_void_p()
@@ -321,8 +330,9 @@ class Scene:
         self.ctx.check(lib().gdem_scene_step_record(self.h, ctypes.byref(view)), "gdem_scene_step_record")
         return d
 
-    def set_timing(self, on=True):
-        self.ctx.check(lib().gdem_scene_set_timing(self.h, 1 if on else 0), "set_timing")
+    def set_timing(self, every=1):
+        """Time the next step() calls; every: sample the force kernel of every n-th step (0 = off)."""
+        self.ctx.check(lib().gdem_scene_set_timing(self.h, int(every)), "set_timing")
 
     def timing(self):
         t = ctypes.c_double()

@@ -160,24 +160,31 @@ def helical_chute(seed=3, n=1000000, r0=0.0005, spread=0.1, density=2500., young
                   mu=0.5, dt=2e-6, steps=10000, skin_factor=0.1, ntri=20000):
     """1,000,000 spheres (r ~ U[0.9, 1.1] x 0.5 mm) resting in layers along a
     4-turn helical chute of radius 0.2 m, pitch 0.1 m and width 0.05 m meshed as
-    20,000 triangles.  Particles sit on a jittered lattice that follows the
-    ramp, a few layers deep, so they flow down the chute together."""
+    20,000 triangles.  Particles sit on a jittered face-centred lattice that
+    follows the ramp, a few layers deep, so they settle and flow down the
+    chute together."""
     rng = np.random.default_rng(seed)
     verts, tilt, z_floor = helical_chute_mesh(seed=seed, ntri=ntri)
     rmax = r0 * (1 + spread)
     radius = rng.uniform(r0 * (1 - spread), rmax, size=n)
-    sp = 2.2 * rmax
+    sp = 2.05 * rmax
     rin, rout = 0.2 - 0.025, 0.2 + 0.025
     clear = 1.1 * rmax
-    rhos = np.arange(rin + clear, rout - clear + 1e-12, sp)
+    # square layers stacked face-centred (each layer sits in the hollows of
+    # the one below: offset half a spacing, rise spacing/sqrt(2) plus a margin
+    # for the ramp's slope and tilt across that offset)
+    rhos = np.arange(rin + clear, rout - clear - 0.5 * sp + 1e-12, sp)
     dth = sp / rin
     total = 2 * math.pi * 4
-    ths = np.arange(dth, total - dth, dth)
+    ths = np.arange(dth, total - 2 * dth, dth)
     nlayer = int(math.ceil(n / (len(rhos) * len(ths))))
     L, T, Rr = np.meshgrid(np.arange(nlayer), ths, rhos, indexing="ij")
     L, T, Rr = L.ravel()[:n], T.ravel()[:n], Rr.ravel()[:n]
+    odd = (L % 2).astype(np.float64)
+    T = T + odd * 0.5 * dth
+    Rr = Rr + odd * 0.5 * sp
     zf = z_floor(T, Rr)
-    z = zf + 1.1 * rmax + L * sp
+    z = zf + clear + L * sp * (1.0 / math.sqrt(2.0) + 0.15)
     j = 0.5 * (sp - 2 * rmax) * 0.8
     x = np.stack([Rr * np.cos(T), Rr * np.sin(T), z], axis=1)
     x = x + rng.uniform(-j, j, size=x.shape)

@@ -81,7 +81,11 @@ typedef struct {
   int wall_cap;     /* wall candidates per particle (0 = 24) */
   int hist_cap;     /* pair contact history slots (0 = 16) */
   int whist_cap;    /* wall contact history slots (0 = 8) */
-  int graph_steps;  /* steps per captured hipGraph (0 = no capture) */
+  int graph_steps;  /* 0: direct launches, list rebuild as one cooperative kernel;
+                       > 0: launch-chain rebuild captured in hipGraphs of this many steps;
+                       < 0: launch-chain rebuild, direct launches (per-phase profiling) */
+  int wall_broadphase; /* 0: static per-cell triangle table when it fits, else BVH;
+                          1: BVH traversal; 2: per-cell table (error if it does not fit) */
 } gdem_params;
 
 /* Host download view (original particle order). Any pointer may be NULL. */
@@ -182,11 +186,13 @@ void gdem_scene_destroy(gdem_scene* scene);
 int gdem_scene_debug(gdem_scene* scene, gdem_debug_view* view);
 /* Runs one step that also records every contact's forces. */
 int gdem_scene_step_record(gdem_scene* scene, gdem_contact_view* view);
-/* Device wall-clock of the last gdem_scene_step call's kernels (ms), and of
- * the force kernel alone, measured with HIP events on the context stream. */
+/* Device time of the last gdem_scene_step call (ms) and the summed time of
+ * the force-kernel launches it sampled, with the number sampled; measured
+ * with HIP events on the context stream. */
 int gdem_scene_timing(gdem_scene* scene, double* total_ms, double* force_ms, int* force_launches);
-/* Enables (1) or disables (0) HIP-event timing of the force kernel. */
-int gdem_scene_set_timing(gdem_scene* scene, int on);
+/* every > 0: time the whole call and the force kernel of every `every`-th
+ * step (direct launches, no graph); 0 disables timing. */
+int gdem_scene_set_timing(gdem_scene* scene, int every);
 
 /* ---- primitives exposed for parity tests ---------------------------------- */
 /* Stable LSD radix sort of (key, iota) on the device: the order
@@ -196,6 +202,10 @@ int gdem_radix_sort(gdem_ctx* ctx, int n, const uint32_t* keys, uint32_t* keys_o
  * one triangle of a mesh: out[np*4] = (nx, ny, nz, dist), hit[np]. */
 int gdem_tri_distance(gdem_ctx* ctx, const gdem_mesh_desc* mesh, int tri, int np,
                       const double* points, const double* rsq, double* out, int* hit);
+/* Plain device-to-device streaming copy of `bytes` bytes, `reps` times; the
+ * average duration per copy (ms).  Calibrates HBM byte counters and gives the
+ * achievable copy bandwidth. */
+int gdem_copy_probe(gdem_ctx* ctx, int64_t bytes, int reps, double* ms_out);
 /* Device FP64 sqrt and division on arrays (checks correct rounding). */
 int gdem_fp64_probe(gdem_ctx* ctx, int n, const double* a, const double* b, double* sqrt_out,
                     double* div_out);

@@ -190,23 +190,30 @@ def test_polydisperse_bed_trajectory_bitwise(ctx):
     assert st["errors"] == 0 and st["rebuilds"] == orc.get("rebuilds")
 
 
-def test_spiral_mesh_walls_bitwise(ctx):
+@pytest.mark.parametrize("broadphase", [1, 2])
+def test_spiral_mesh_walls_bitwise(ctx, broadphase):
     # 2,498-triangle reference mesh with obtuse facets: wall lists, vertex/edge
-    # dedup and wall histories
+    # dedup and wall histories.  Spheres start just above facet centroids.
     v, counts = scenes.read_ascii_stl(golden("spiral.stl"))
     rng = np.random.default_rng(11)
-    lo, hi = v.reshape(-1, 3).min(0), v.reshape(-1, 3).max(0)
-    pts = v.reshape(-1, 3)[rng.choice(len(v) * 3, 3000, replace=False)]
-    x = pts + rng.normal(size=pts.shape) * 0.01
-    r = np.full(len(x), 0.012)
+    tri = v.reshape(-1, 3, 3)
+    cen = tri.mean(axis=1)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    r0 = 0.012
+    pick = rng.choice(len(tri), 1500, replace=False)
+    x = cen[pick] + nrm[pick] * r0 * 1.02
     keep = np.ones(len(x), bool)
     for i in range(len(x)):  # drop overlapping starts
         if keep[i]:
             d = np.linalg.norm(x[i + 1:] - x[i], axis=1)
-            keep[i + 1:][d < 0.025] = False
-    x, r = x[keep], r[keep]
+            keep[i + 1:][d < 2.05 * r0] = False
+    x = x[keep]
+    r = np.full(len(x), r0)
+    lo, hi = v.reshape(-1, 3).min(0), v.reshape(-1, 3).max(0)
     bbox = scenes.bbox_rule(v, [lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]])
-    params = dict(dt=1e-5, mu=0.3, g=(0., -9.81, 0.), bbox=bbox, cell=0.048, skin=0.012, refresh=0.006)
+    params = dict(dt=1e-5, mu=0.3, g=(0., -9.81, 0.), bbox=bbox, cell=0.048, skin=0.012, refresh=0.006,
+                  wall_cap=48, whist_cap=32, wall_broadphase=broadphase)
     spec = scenes.SceneSpec("spiral", x, r, v, params, 1000., 1e7, 0.3)
     say(f"scene spiral n={len(x)}")
     m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
@@ -216,8 +223,11 @@ def test_spiral_mesh_walls_bitwise(ctx):
     for chunk in range(4):
         sc.step(500)
         orc.step(500)
+        st = sc.status(check=False)
+        assert st["errors"] == 0 and orc.get("overflow") == 0, st
         assert_state_equal(sc, orc, f"step {(chunk + 1) * 500}")
     assert_history_equal(sc, orc)
+    assert sc.status()["mean_contacts"] > 0.2
 
 
 def test_inactive_particles_and_out_of_bounds(ctx):
@@ -244,7 +254,7 @@ def test_mixed_precision_force_error(ctx):
     say("scene polydisperse_bed mixed vs fp64")
     m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
     s64 = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, spec.params)
-    s64.step(4000)
+    s64.step(20000)
     st = s64.download()
     # same state into both precisions, one recorded step each
     a = Scene(ctx, st["x"], spec.r, m, k, e, spec.verts, spec.params, v=st["v"], angv=st["angv"],
@@ -325,6 +335,25 @@ def test_chute_full_size_properties(ctx, chute, precision):
     b.step(300)
     db = b.download()
     assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["angv"], db["angv"])
+    a.close()
+    b.close()
+
+
+def test_chute_wall_table_matches_bvh(ctx, chute):
+    # static per-cell table and BVH traversal give identical candidate lists
+    spec = chute
+    say("scene helical_chute table vs bvh")
+    m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
+    a = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, dict(spec.params, wall_broadphase=2,
+                                                             precision=GDEM_MIXED))
+    b = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, dict(spec.params, wall_broadphase=1,
+                                                             precision=GDEM_MIXED))
+    a.step(2000)
+    b.step(2000)
+    da, db = a.debug(), b.debug()
+    assert np.array_equal(da["nwall"], db["nwall"]) and np.array_equal(da["wall"], db["wall"])
+    assert da["nwall"].sum() > 10000
+    assert np.array_equal(a.download()["x"], b.download()["x"])
     a.close()
     b.close()
 
