@@ -231,7 +231,11 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   // radix-sort / cooperative grid: one workgroup per chunk of >= 256 keys
   SimParams& P = sc->P;
   const int want = (N + 255) / 256;
-  const int limit = sc->chain ? 0 : coop_grid_limit<H>();
+  // The cooperative grid stays well below the co-residency the occupancy
+  // query reports (half of it, at most 1024 workgroups): every workgroup must
+  // be resident at once for the grid barriers, and the rebuild phases are
+  // memory-bound loops that do not need more.
+  const int limit = sc->chain ? 0 : std::min(1024, coop_grid_limit<H>() / 2);
   sc->coop = limit > 0;
   P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want))
                          : std::max(1, std::min(1024, (N + 2047) / 2048));

@@ -64,6 +64,37 @@ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
   return (spread10(x) << 2) | (spread10(y) << 1) | spread10(z);
 }
 
+// Non-temporal (streaming) accesses for data a step touches once, so the L2
+// keeps the neighbour positions and velocities that tiles gather.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T v, T* p) {
+  __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ double4 ld_nt(const double4* p) {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_double4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st_nt(double4 a, double4* p) {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  const v4 v = {a.x, a.y, a.z, a.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4*>(p));
+}
+__device__ __forceinline__ void st_nt(float4 a, float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = {a.x, a.y, a.z, a.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4*>(p));
+}
+
 // Last-workgroup detection: returns true in exactly one thread (thread 0 of
 // the last workgroup to finish).  No waiting is involved.  Callers use it only
 // to flip control words that every workgroup read at its start, so no memory
@@ -186,7 +217,7 @@ struct HistLds {
     marks = 0;
 #pragma unroll
     for (int q = 0; q < CAP; ++q)
-      if (q < nh) k[q * 256] = gkey[(size_t)q * N + s];
+      if (q < nh) k[q * 256] = ld_nt(gkey + (size_t)q * N + s);
   }
   __device__ __forceinline__ int find(int key) const {
     for (int q = 0; q < n; ++q)
@@ -221,23 +252,20 @@ struct HistScan {
 // history).  The slot of `key` is found or appended (d_calc_tangential_force
 // :630-653); T is the force type and FP64 keeps the reference's operations.
 template <class T, class Row>
-__device__ __forceinline__ V3<T> tangential(Row& row, uint32_t& err, uint32_t ovf_bit, int key, V3<T> fn,
-                                            V3<T> vt, V3<T> n, T eta, T k, T mu, T dt) {
+__device__ __forceinline__ V3<T> tangential_at(Row& row, uint32_t& err, uint32_t ovf_bit, int key, int slot,
+                                               V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu,
+                                               T dt) {
   using H4 = typename Row::H4;
-  int slot = row.find(key);
-  V3<T> old{T(0), T(0), T(0)};
   if (slot < 0) {
+    // new contact: appended in list order (old is zero)
     if (row.n < row.cap) {
       slot = row.n;
-      row.gkey[(size_t)slot * row.N + row.s] = key;
+      st_nt(key, row.gkey + (size_t)slot * row.N + row.s);
       row.remember(key);
       ++row.n;
     } else {
       err |= ovf_bit;
     }
-  } else {
-    const H4 d = row.gd[(size_t)slot * row.N + row.s];
-    old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
   }
   if (slot >= 0) row.marks |= (uint64_t)1 << slot;
   V3<T> dn{old.x + vt.x * dt, old.y + vt.y * dt, old.z + vt.z * dt};
@@ -266,9 +294,23 @@ __device__ __forceinline__ V3<T> tangential(Row& row, uint32_t& err, uint32_t ov
     o.y = (decltype(o.y))dn.y;
     o.z = (decltype(o.z))dn.z;
     o.w = 0;
-    row.gd[(size_t)slot * row.N + row.s] = o;
+    st_nt(o, row.gd + (size_t)slot * row.N + row.s);
   }
   return ft;
+}
+
+// Slot lookup + history read, then tangential_at.
+template <class T, class Row>
+__device__ __forceinline__ V3<T> tangential(Row& row, uint32_t& err, uint32_t ovf_bit, int key, V3<T> fn,
+                                            V3<T> vt, V3<T> n, T eta, T k, T mu, T dt) {
+  using H4 = typename Row::H4;
+  const int slot = row.find(key);
+  V3<T> old{T(0), T(0), T(0)};
+  if (slot >= 0) {
+    const H4 d = ld_nt(row.gd + (size_t)slot * row.N + row.s);
+    old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
+  }
+  return tangential_at<T>(row, err, ovf_bit, key, slot, old, fn, vt, n, eta, k, mu, dt);
 }
 
 // Slots from the first one without a contact this step are dropped
@@ -305,9 +347,17 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // ---------------------------------------------------------------------------
 // fused contact forces + integration for one step
 // ---------------------------------------------------------------------------
+// Touching pairs evaluated per round of pass 2 (their loads overlap).
+#ifndef GDEM_PAIRS_PER_ROUND
+#define GDEM_PAIRS_PER_ROUND 1
+#endif
+constexpr int PAIRS_PER_ROUND = GDEM_PAIRS_PER_ROUND;
+
 // PRow/WRow: history row types for pairs and walls.  Pair candidates are
-// fetched four at a time (indices, then positions) so their latencies
-// overlap; contacts are still evaluated in list order.
+// screened first (indices and positions fetched eight at a time so their
+// latencies overlap, FP64 overlap test), then the touching ones are evaluated
+// in list order.  The occupancy bound of 4 waves per SIMD costs the FP64 form
+// a few spilled registers and pays for itself (measured on MI355X).
 template <int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
@@ -330,7 +380,7 @@ struct RowCap<HistLds<H, CAP>> {
 };
 
 template <class T, class H, bool REC, class PRow, class WRow>
-__global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
+__global__ __launch_bounds__(256, 4) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
   using H4 = typename Vec4<H>::type;
   __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int tx = threadIdx.x;
@@ -348,11 +398,11 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
     const double4 v = vin[s];
     const double4 w = win[s];
     if (p.w < 0.) {
-      B.pos[cur ^ 1][s] = p;
-      B.vel[cur ^ 1][s] = v;
-      B.angv[cur ^ 1][s] = w;
+      st_nt(p, B.pos[cur ^ 1] + s);
+      st_nt(v, B.vel[cur ^ 1] + s);
+      st_nt(w, B.angv[cur ^ 1] + s);
     } else {
-      const H4 mt = B.mat[hc][s];
+      const H4 mt = ld_nt(B.mat[hc] + s);
       const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
       const double invmoi = (double)mt.w;
       const double ri = p.w, invm = v.w;
@@ -363,8 +413,9 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
       V3<T> f{T(0), T(0), T(0)}, mom{T(0), T(0), T(0)};
       int nrec = 0;
       const int self_id = (int)w.w;
-      const int nnb = B.nnb[s];
-      const int nwl = B.nwl[s];
+      const int nnb = ld_nt(B.nnb + s);
+      const int nwl = ld_nt(B.nwl + s);
+      const double4 rf = ld_nt(B.ref + s);  // position at the last list build
 
       // ---- particle pairs -------------------------------------------------
       {
@@ -375,28 +426,64 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
         row.s = s;
         row.N = N;
         row.cap = P.hist_cap;
-        row.load(B.nh[hc][s]);
-        for (int q0 = 0; q0 < nnb; q0 += 4) {
-          // fetch four candidates (indices, then positions) before using any
-          int j0 = B.nbr[(size_t)q0 * N + s];
-          int j1 = (q0 + 1 < nnb) ? B.nbr[(size_t)(q0 + 1) * N + s] : -1;
-          int j2 = (q0 + 2 < nnb) ? B.nbr[(size_t)(q0 + 2) * N + s] : -1;
-          int j3 = (q0 + 3 < nnb) ? B.nbr[(size_t)(q0 + 3) * N + s] : -1;
-          double4 p0 = pin[j0];
-          double4 p1 = j1 >= 0 ? pin[j1] : make_double4(0., 0., 0., -1.);
-          double4 p2 = j2 >= 0 ? pin[j2] : make_double4(0., 0., 0., -1.);
-          double4 p3 = j3 >= 0 ? pin[j3] : make_double4(0., 0., 0., -1.);
-          for (int u = 0; u < 4; ++u) {
-            const int j = j0;
-            const double4 pj = p0;
-            j0 = j1; j1 = j2; j2 = j3; j3 = -1;
-            p0 = p1; p1 = p2; p2 = p3;
-            if (j < 0) break;
+        row.load(ld_nt(B.nh[hc] + s));
+        // pass 1: which candidates overlap (FP64 test of the reference, list
+        // order); candidates are fetched eight at a time so latencies overlap
+        uint64_t hits = 0;
+        for (int q0 = 0; q0 < nnb; q0 += 8) {
+          int jv[8];
+          double4 cv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) jv[u] = (q0 + u < nnb) ? B.nbr[(size_t)(q0 + u) * N + s].x : -1;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pin[jv[u]] : make_double4(0., 0., 0., -1.);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const double4 pj = cv[u];
             if (pj.w < 0.) continue;
             const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
             const double distsq = dx * dx + dy * dy + dz * dz;
             const double R = ri + pj.w;
-            if (!(distsq < R * R)) continue;
+            if (distsq < R * R) hits |= (uint64_t)1 << (q0 + u);
+          }
+        }
+        // pass 2: the touching pairs in list order, two at a time.  Their
+        // history slots are looked up first (new slots are still appended in
+        // list order), so position, velocity, spin and history of both are
+        // fetched in one round.
+        while (hits) {
+          const int qa = __ffsll((unsigned long long)hits) - 1;
+          hits &= hits - 1;
+          const int qb = (PAIRS_PER_ROUND > 1 && hits) ? __ffsll((unsigned long long)hits) - 1 : -1;
+          if (qb >= 0) hits &= hits - 1;
+          int2 e0 = B.nbr[(size_t)qa * N + s];
+          int2 e1 = qb >= 0 ? B.nbr[(size_t)qb * N + s] : make_int2(-1, -1);
+          int s0 = row.find(e0.y);
+          int s1 = e1.x >= 0 ? row.find(e1.y) : -1;
+          double4 pj0 = pin[e0.x], vj0 = vin[e0.x], wj0 = win[e0.x];
+          H4 d0 = s0 >= 0 ? ld_nt(row.gd + (size_t)s0 * N + s) : H4{};
+          double4 pj1 = make_double4(0., 0., 0., 0.), vj1 = pj1, wj1 = pj1;
+          H4 d1{};
+          if (e1.x >= 0) {
+            pj1 = pin[e1.x];
+            vj1 = vin[e1.x];
+            wj1 = win[e1.x];
+            if (s1 >= 0) d1 = ld_nt(row.gd + (size_t)s1 * N + s);
+          }
+          for (int u = 0; u < 2; ++u) {
+            if (u == 1) {
+              if (e1.x < 0) break;
+              e0 = e1;
+              s0 = s1;
+              pj0 = pj1;
+              vj0 = vj1;
+              wj0 = wj1;
+              d0 = d1;
+            }
+            const double4 pj = pj0, vj = vj0, wj = wj0;
+            const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+            const double distsq = dx * dx + dy * dy + dz * dz;
+            const double R = ri + pj.w;
             const double dist = sqrt(distsq);
             const double delMagD = R - dist;
             V3<T> n;
@@ -407,8 +494,6 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
               n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
             }
             const T delMag = (T)delMagD;
-            const double4 vj = vin[j];
-            const double4 wj = win[j];
             const V3<T> del = scal(-delMag, n);
             const V3<T> vr{vi.x - (T)vj.x, vi.y - (T)vj.y, vi.z - (T)vj.z};
             const T vrn = dot(vr, n);
@@ -426,8 +511,9 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
             f.x += fn.x;
             f.y += fn.y;
             f.z += fn.z;
-            const int key = (int)wj.w;  // original id of j
-            const V3<T> ft = tangential<T>(row, err, GDEM_ERR_HIST_OVERFLOW_BIT, key, fn, vt, n, eta, k, mu, dtT);
+            const int key = e0.y;  // original id of j
+            const V3<T> ft = tangential_at<T>(row, err, GDEM_ERR_HIST_OVERFLOW_BIT, key, s0,
+                                                 V3<T>{(T)d0.x, (T)d0.y, (T)d0.z}, fn, vt, n, eta, k, mu, dtT);
             f.x += ft.x;
             f.y += ft.y;
             f.z += ft.z;
@@ -448,7 +534,7 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
             }
           }
         }
-        B.nh[hc][s] = truncate_history(row.n, row.marks);
+        st_nt(truncate_history(row.n, row.marks), B.nh[hc] + s);
       }
 
       // ---- walls ------------------------------------------------------------
@@ -460,13 +546,13 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
         row.s = s;
         row.N = N;
         row.cap = P.whist_cap;
-        row.load(B.nw[hc][s]);
+        row.load(ld_nt(B.nw[hc] + s));
         int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
         int nve = 0;
         const double rsq = ri * ri;
         const T eta = etac * sqrtm;
         for (int q = 0; q < nwl; ++q) {
-          const int t = B.wl[(size_t)q * N + s];
+          const int t = ld_nt(B.wl + (size_t)q * N + s);
           const TriHit h = tri_distance(B.tri + t, p.x, p.y, p.z, rsq);
           if (!(h.dist < ri)) continue;
           if (h.hit != -1) {
@@ -518,7 +604,7 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
             }
           }
         }
-        B.nw[hc][s] = truncate_history(row.n, row.marks);
+        st_nt(truncate_history(row.n, row.marks), B.nw[hc] + s);
       }
       if constexpr (REC) rec.count[self_id] = nrec;
 
@@ -546,16 +632,15 @@ __global__ __launch_bounds__(256) void k_step(SimParams P, Bufs<H> B, Rec<T> rec
                        po.y > P.bmax[1] || po.z < P.bmin[2] || po.z > P.bmax[2];
       po.w = out ? -ri : ri;
       if (!out) {
-        const double4 rf = B.ref[s];
         const double ex = po.x - rf.x, ey = po.y - rf.y, ez = po.z - rf.z;
         refresh = (ex * ex + ey * ey + ez * ez) > P.threshSq;
       }
       if (!(isfinite(po.x) && isfinite(po.y) && isfinite(po.z) && isfinite(vo.x) &&
             isfinite(vo.y) && isfinite(vo.z)))
         err |= GDEM_ERR_NAN_BIT;
-      B.pos[cur ^ 1][s] = po;
-      B.vel[cur ^ 1][s] = vo;
-      B.angv[cur ^ 1][s] = wo;
+      st_nt(po, B.pos[cur ^ 1] + s);
+      st_nt(vo, B.vel[cur ^ 1] + s);
+      st_nt(wo, B.angv[cur ^ 1] + s);
     }
   }
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
@@ -804,7 +889,7 @@ __device__ __forceinline__ void pairlist_one(const SimParams& P, const Bufs<H>& 
       }
     }
   }
-  for (int q = 0; q < n; ++q) B.nbr[(size_t)q * N + s] = cand[q];
+  for (int q = 0; q < n; ++q) B.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
   B.nnb[s] = n;
   if (ovf) atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_NB_OVERFLOW_BIT);
 }
@@ -953,19 +1038,18 @@ __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
   if (s < P.N) cells_one(P, B, s);
 }
 
+// Pair and wall lists of the sorted particles, at full occupancy (both the
+// chain and the cooperative rebuild end here).  The last workgroup clears the
+// refresh flag, which ends the rebuild.
 template <class H>
-__global__ __launch_bounds__(256) void k_pairlist(SimParams P, Bufs<H> B) {
+__global__ __launch_bounds__(256) void k_lists(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < P.N) pairlist_one(P, B, s, B.ctl[CTL_CUR], B.ctl[CTL_HCUR], B.ctl[CTL_NACT]);
-}
-
-// The last workgroup clears the refresh flag, which ends the rebuild.
-template <class H>
-__global__ __launch_bounds__(256) void k_walllist(SimParams P, Bufs<H> B) {
-  if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < P.N) walllist_one(P, B, s, B.ctl[CTL_CUR], B.ctl[CTL_NACT]);
+  if (s < P.N) {
+    const int cur = B.ctl[CTL_CUR], hc = B.ctl[CTL_HCUR], nact = B.ctl[CTL_NACT];
+    pairlist_one(P, B, s, cur, hc, nact);
+    walllist_one(P, B, s, cur, nact);
+  }
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -973,11 +1057,13 @@ __global__ __launch_bounds__(256) void k_walllist(SimParams P, Bufs<H> B) {
   }
 }
 
-// ---- the whole rebuild in one cooperative kernel ---------------------------
-// Launched every step with a co-resident grid; returns at once (before any
-// grid barrier) when the refresh flag is clear, so an idle step costs one
-// launch.  Every workgroup reads the control words before the first barrier;
-// workgroup 0 updates them after the last one.
+// ---- the global rebuild phases in one cooperative kernel -------------------
+// Keys, the four radix passes, the permutation and the cell table need grid
+// barriers; they run in one cooperative kernel launched every step with a
+// co-resident grid, which returns at once (before any barrier) when the
+// refresh flag is clear.  Every workgroup reads the control words before the
+// first barrier; workgroup 0 flips the buffers at the end.  The lists follow
+// in k_lists.
 template <class H>
 __global__ __launch_bounds__(256) void k_rebuild_coop(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
@@ -1025,17 +1111,9 @@ __global__ __launch_bounds__(256) void k_rebuild_coop(SimParams P, Bufs<H> B) {
   for (int s = first; s < N; s += stride) permute_one(P, B, s, cur, hc);
   grid.sync();
   for (int s = first; s < N; s += stride) cells_one(P, B, s);
-  grid.sync();
-  const int nact = __hip_atomic_load(&B.ctl[CTL_NACT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int s = first; s < N; s += stride) {
-    pairlist_one(P, B, s, cur ^ 1, hc ^ 1, nact);
-    walllist_one(P, B, s, cur ^ 1, nact);
-  }
   if (b == 0 && threadIdx.x == 0) {
     B.ctl[CTL_CUR] = cur ^ 1;
     B.ctl[CTL_HCUR] = hc ^ 1;
-    B.ctl[CTL_REBUILDS] += 1;
-    __hip_atomic_store(&B.ctl[CTL_REFRESH], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1074,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
   D.morton[id] = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
   const int nn = B.nnb[s];
   D.nnb[id] = nn;
-  for (int q = 0; q < nn; ++q) D.nb[(size_t)id * P.nb_cap + q] = B.pid[hc][B.nbr[(size_t)q * N + s]];
+  for (int q = 0; q < nn; ++q) D.nb[(size_t)id * P.nb_cap + q] = B.nbr[(size_t)q * N + s].y;
   const int nw = B.nwl[s];
   D.nwall[id] = nw;
   for (int q = 0; q < nw; ++q) D.wall[(size_t)id * P.wall_cap + q] = B.wl[(size_t)q * N + s];
@@ -1214,7 +1292,11 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
     SimParams p = P;
     Bufs<H> b = B;
     void* args[] = {&p, &b};
-    return hipLaunchCooperativeKernel((const void*)k_rebuild_coop<H>, dim3(P.rs_blocks), dim3(256), args, 0, st);
+    const hipError_t e =
+        hipLaunchCooperativeKernel((const void*)k_rebuild_coop<H>, dim3(P.rs_blocks), dim3(256), args, 0, st);
+    if (e != hipSuccess) return e;
+    k_lists<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B);
+    return hipGetLastError();
   }
   const int g = blocks_for(P.N, 256);
   k_keys<H><<<g, 256, 0, st>>>(P, B);
@@ -1222,8 +1304,7 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
                     B.ctl, true, st);
   k_permute<H><<<g, 256, 0, st>>>(P, B);
   k_cells<H><<<g, 256, 0, st>>>(P, B);
-  k_pairlist<H><<<g, 256, 0, st>>>(P, B);
-  k_walllist<H><<<g, 256, 0, st>>>(P, B);
+  k_lists<H><<<g, 256, 0, st>>>(P, B);
   return hipGetLastError();
 }
 

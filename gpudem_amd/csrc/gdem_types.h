@@ -92,7 +92,7 @@ struct Bufs {
   H4* wd[2];         // [whist_cap][N]
   int* nw[2];
   double4* ref;      // positions at the last list build
-  int* nbr;          // pair candidates (internal index), slot-major [nb_cap][N]
+  int2* nbr;         // pair candidates (internal index, original id), slot-major [nb_cap][N]
   int* nnb;
   int* wl;           // wall candidates (triangle id) [wall_cap][N]
   int* nwl;
