@@ -1,8 +1,10 @@
 """Throughput of the contact step on the 1,000,000-sphere helical chute.
 
 Workload: 1,000,000 spheres (r ~ U[0.9,1.1] x 0.5 mm) on a 4-turn helical
-chute meshed as 20,000 triangles, FP32 forces with FP64 integration, LBVH wall
-broadphase, full tangential history, dt = 2e-6 s.  W warm-up steps let the bed
+chute meshed as 20,000 triangles, FP32 forces with FP64 integration, full
+tangential history, dt = 2e-6 s.  Wall candidates come from a static per-cell
+triangle table built once from the mesh (it yields the same candidate lists as
+the BVH traversal, which stays selectable with --wall-broadphase 1).  W warm-up steps let the bed
 settle into contact, then K steps are timed with HIP events; the force kernel
 (the dominant one) of every n-th timed step is bracketed by its own event pair
 on the engine's stream.
@@ -12,7 +14,9 @@ all state resident in HBM.  `roofline` prices the force kernel's algorithmic
 bytes per launch -- 192 + 4 (n_nb + n_w) + 32 c bytes per active particle for
 the mixed form, 212 + 4 (n_nb + n_w) + 56 c for FP64, with the candidate and
 contact counts measured at the end of the window -- against the 8 TB/s HBM
-peak, and reports a plain device copy's bandwidth beside it.  `cpu_baseline`
+peak, and reports a plain device copy's bandwidth beside it; `traffic` is the
+measured HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary.  `cpu_baseline`
 times the C++/OpenMP oracle (a restatement of the reference's
 cpu_dem_verlet_verlet) on the same settled state.
 """
@@ -26,6 +30,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_chute1M_traffic.json")
+
+
+def measured_traffic(n_active):
+    """HBM bytes per force-kernel launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (calibrated on a copy of known size), scaled to n_active."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    per = t["fetch_bytes_per_particle_step"] + t["write_bytes_per_particle_step"]
+    return per * n_active, t
 
 
 def algorithmic_bytes(mixed, nb, nw, c):
@@ -37,6 +54,7 @@ def make_scene(ctx, spec, precision, graph_steps, state=None):
     from gpudem_amd import Scene, engine
     m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
     params = dict(spec.params, precision=precision, graph_steps=graph_steps)
+    params.setdefault("wall_broadphase", 0)
     if state is None:
         return Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, params)
     return Scene(ctx, state["x"], spec.r, m, k, e, spec.verts, params, v=state["v"], angv=state["angv"],
@@ -93,13 +111,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=10000)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="> 0: launch-chain rebuild captured in graphs of this many steps; "
                          "< 0: launch-chain rebuild, direct launches")
+    ap.add_argument("--wall-broadphase", type=int, default=0,
+                    help="0: per-cell table when it fits, 1: BVH traversal, 2: per-cell table")
     ap.add_argument("--sample-every", type=int, default=10,
                     help="bracket the force kernel of every n-th timed step with HIP events")
     ap.add_argument("--save-state", help="write the mixed run's final state to this .npz")
@@ -114,6 +134,7 @@ def main():
 
     from gpudem_amd import GDEM_FP64, GDEM_MIXED, Context, scenes
     spec = scenes.helical_chute(n=args.particles)
+    spec.params["wall_broadphase"] = args.wall_broadphase
     print(f"# workload helical_chute n={spec.n} ntri={len(spec.verts)}", file=sys.stderr, flush=True)
     state = None
     if args.load_state:
@@ -132,6 +153,7 @@ def main():
         return
     m = summarize(mixed, args.steps, True)
     s1 = mixed["stats1"]
+    traffic, tsrc = measured_traffic(m["nact"])
     print(f"# mixed: {m['ms_per_step']:.4f} ms/step, force kernel {m['force_avg_s'] * 1e6:.1f} us, "
           f"B={m['B']:.1f} B/particle-step, rebuilds {m['rebuilds']}", file=sys.stderr, flush=True)
     out = {
@@ -150,12 +172,14 @@ def main():
         "config": {"workload": "helical_chute_1M_20k_tris_mixed", "particles": spec.n,
                    "triangles": int(len(spec.verts)), "dt": spec.params["dt"],
                    "skin": spec.params["skin"], "cell": spec.params["cell"],
-                   "rebuild": "cooperative" if args.graph_steps == 0 else "chain"},
+                   "rebuild": "cooperative" if args.graph_steps == 0 else "chain",
+                   "wall_broadphase": ["table", "bvh", "table"][args.wall_broadphase]},
         "list_stats": {"mean_nb": s1["mean_nb"], "mean_wall": s1["mean_wall"],
                        "mean_contacts": s1["mean_contacts"], "rebuilds_in_window": m["rebuilds"],
                        "n_active": m["nact"]},
         "roofline": {"bound": "hbm", "achieved": m["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": m["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": m["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": tsrc["source"] if tsrc else None,
                      "kernel": "k_step<float,float>", "bytes_per_particle_step": m["B"],
                      "avg_launch_us": m["force_avg_s"] * 1e6, "sampled_launches": mixed["launches"],
                      "copy_bw": copy_bw, "frac_of_copy": m["achieved"] / copy_bw},
