@@ -5,8 +5,8 @@ ABI in include/gdem.h); this package holds its ctypes binding, scene
 producers and the build script.
 """
 from . import scenes  # noqa: F401
-from .engine import (GDEM_FP64, GDEM_MIXED, Context, GdemError, Scene, bbox_rule,  # noqa: F401
+from .engine import (GDEM_FP64, GDEM_MIXED, Batch, Context, GdemError, Scene, bbox_rule,  # noqa: F401
                      lib, material)
 
-__all__ = ["Context", "Scene", "GdemError", "GDEM_FP64", "GDEM_MIXED", "material", "bbox_rule",
+__all__ = ["Context", "Scene", "Batch", "GdemError", "GDEM_FP64", "GDEM_MIXED", "material", "bbox_rule",
            "lib", "scenes"]

@@ -10,55 +10,9 @@
 #include <vector>
 
 #include "../../include/gdem.h"
+#include "gdem_internal.h"
 #include "gdem_kernels.h"
 #include "gdem_mesh.h"
-
-struct gdem_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-};
-
-namespace gdem {
-namespace {
-
-int fail(gdem_ctx* c, int code, const std::string& what) {
-  if (c) c->err = what;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail((ctx), e_ == hipErrorOutOfMemory ? GDEM_E_OOM : GDEM_E_HIP,           \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
-  } while (0)
-
-// Owns every device allocation of a scene.
-struct DevArena {
-  std::vector<void*> ptrs;
-  ~DevArena() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-  template <class T>
-  hipError_t alloc(T** out, size_t n) {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
-    if (e == hipSuccess) {
-      ptrs.push_back(p);
-      *out = static_cast<T*>(p);
-    }
-    return e;
-  }
-};
-
-struct SceneBase {
-  virtual ~SceneBase() = default;
-};
-
-}  // namespace
-}  // namespace gdem
 
 struct gdem_scene {
   gdem_ctx* ctx = nullptr;

@@ -1,0 +1,374 @@
+// gfx950 kernels of the batched engine: many independent scenes stepped by
+// one launch per phase.
+//
+// Every scene owns a 64-aligned segment of slots, so each wave serves exactly
+// one scene and reads that scene's constants with scalar loads.  A step is
+//   kb_bin -> kb_scan -> kb_permute -> kb_lists   (only scenes whose refresh
+//                                                  flag is set do any work)
+//   kb_step                                       (forces + integration of
+//                                                  every running scene)
+// and is identical for every step, so a run of steps is captured once into a
+// hipGraph and replayed.  A rebuild re-bins one scene by cell with a counting
+// sort (atomic per-cell ranks, one workgroup scanning the scene's cells, a
+// scatter into the other buffer half), so it never touches other scenes.  The
+// contact step itself is the same device code as the single-scene kernel, and
+// the lists order contacts by particle id, so a scene's trajectory in a batch
+// is bit-identical to the same scene stepped alone.
+#include <hip/hip_runtime.h>
+
+#include "gdem_batch.h"
+#include "gdem_device.h"
+
+namespace gdem {
+
+// The step kernel running while STEP == u raises refresh flags at parity u&1;
+// the rebuild that follows it runs while STEP == u+1 and so reads parity
+// (STEP+1)&1.
+__device__ __forceinline__ int rebuild_parity(int step) { return (step + 1) & 1; }
+
+// Scene of the wave holding slot i if that scene is due for a rebuild, else -1.
+template <class H>
+__device__ __forceinline__ int due_scene(const BBufs<H>& B, int i, int step) {
+  const int w = __builtin_amdgcn_readfirstlane(i >> 6);
+  if (w >= (B.cap >> 6)) return -1;
+  const int sid = __builtin_amdgcn_readfirstlane(B.wave_scene[w]);
+  if (sid < 0) return -1;
+  if (__builtin_amdgcn_readfirstlane(B.sctl[sid].fl[rebuild_parity(step)]) == 0) return -1;
+  return sid;
+}
+
+template <class H>
+__device__ __forceinline__ bool any_due(const BBufs<H>& B, int step) {
+  return __builtin_amdgcn_readfirstlane(B.ctl[BC_ANY0 + rebuild_parity(step)]) != 0;
+}
+
+// Cell of every slot of a due scene (d_calcCellId, BoundingBox.cu:1089-1104)
+// and its rank among the particles of that cell.  Inactive particles, and any
+// position outside the grid, are ranked among the scene's inactive ones.
+template <class H>
+__global__ __launch_bounds__(256) void kb_bin(BBufs<H> B) {
+  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
+  if (!any_due(B, step)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sid = due_scene(B, i, step);
+  if (sid < 0) return;
+  const BScene& S = B.scenes[sid];
+  SlotCtl& C = B.sctl[sid];
+  const int cur = (step & 1) ^ __builtin_amdgcn_readfirstlane(C.q);
+  const double4 p = B.pos[cur][i];
+  bool in = p.w >= 0.;
+  int c = -1;
+  if (in) {
+    const double fx = floor((p.x - S.bmin[0]) * S.invd[0]) + 1;
+    const double fy = floor((p.y - S.bmin[1]) * S.invd[1]) + 1;
+    const double fz = floor((p.z - S.bmin[2]) * S.invd[2]) + 1;
+    in = fx >= 0. && fx < (double)S.size[0] && fy >= 0. && fy < (double)S.size[1] && fz >= 0. &&
+         fz < (double)S.size[2];
+    if (in) c = S.cell_base + (S.size[1] * (int)fz + (int)fy) * S.size[0] + (int)fx;
+  }
+  if (in) {
+    B.bin_cell[i] = c;
+    B.bin_rank[i] = atomicAdd(&B.ccount[c], 1);
+  } else {
+    B.bin_cell[i] = -1;
+    B.bin_rank[i] = atomicAdd(&C.inact, 1);
+  }
+}
+
+// One workgroup per due scene: exclusive scan of the scene's cell counts into
+// the (first, end) slot table, clearing the counters for the next rebuild.
+constexpr int SCAN_PER = 16;
+
+template <class H>
+__global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
+  const int step = B.ctl[BC_STEP];
+  if (B.ctl[BC_ANY0 + rebuild_parity(step)] == 0) return;
+  const int sid = blockIdx.x;
+  SlotCtl& C = B.sctl[sid];
+  if (C.fl[rebuild_parity(step)] == 0) return;
+  const BScene& S = B.scenes[sid];
+  __shared__ int part[256];
+  const int tid = threadIdx.x;
+  const int base = S.cell_base, n = S.ncell;
+  int carry = 0;
+  for (int c0 = 0; c0 < n; c0 += 256 * SCAN_PER) {
+    const int beg = c0 + tid * SCAN_PER;
+    int v[SCAN_PER];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = (beg + k < n) ? B.ccount[base + beg + k] : 0;
+      sum += v[k];
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const int add = tid >= off ? part[tid - off] : 0;
+      __syncthreads();
+      part[tid] += add;
+      __syncthreads();
+    }
+    int run = S.seg0 + carry + part[tid] - sum;
+    const int total = part[255];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      if (beg + k < n) {
+        B.cell_tab[base + beg + k] = make_int2(run, run + v[k]);
+        if (v[k]) B.ccount[base + beg + k] = 0;
+        run += v[k];
+      }
+    }
+    carry += total;
+  }
+  if (tid == 0) {
+    C.nact = carry;
+    C.inact = 0;
+    C.q2 = C.q ^ 1;
+  }
+}
+
+// Scatter every slot of a due scene to its cell-ordered position in the other
+// buffer half: state, material, ids and contact histories (d_swap_ps,
+// BoundingBox.cu:928-1005, here including history rows, stiffness and ids).
+template <class H>
+__global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
+  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
+  if (!any_due(B, step)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sid = due_scene(B, i, step);
+  if (sid < 0) return;
+  const BScene& S = B.scenes[sid];
+  SlotCtl& C = B.sctl[sid];
+  const int q = __builtin_amdgcn_readfirstlane(C.q);
+  const int cur = (step & 1) ^ q;
+  const int c = B.bin_cell[i];
+  const int r = B.bin_rank[i];
+  const int d = c >= 0 ? B.cell_tab[c].x + r : S.seg0 + C.nact + r;
+  if (d < S.seg0 || d >= S.seg0 + S.nseg) {
+    atomicOr(&C.err, GDEM_ERR_NB_OVERFLOW_BIT);  // inconsistent counts: never write outside the segment
+    return;
+  }
+  const size_t N = (size_t)B.cap;
+  B.pos[cur ^ 1][d] = B.pos[cur][i];
+  B.vel[cur ^ 1][d] = B.vel[cur][i];
+  B.angv[cur ^ 1][d] = B.angv[cur][i];
+  B.mat[q ^ 1][d] = B.mat[q][i];
+  B.pid[q ^ 1][d] = B.pid[q][i];
+  B.cell_of[d] = c;
+  const int nh = B.nh[q][i];
+  B.nh[q ^ 1][d] = nh;
+  for (int k = 0; k < nh; ++k) {
+    B.hkey[q ^ 1][k * N + d] = B.hkey[q][k * N + i];
+    B.hd[q ^ 1][k * N + d] = B.hd[q][k * N + i];
+  }
+  const int nw = B.nw[q][i];
+  B.nw[q ^ 1][d] = nw;
+  for (int k = 0; k < nw; ++k) {
+    B.wkey[q ^ 1][k * N + d] = B.wkey[q][k * N + i];
+    B.wd[q ^ 1][k * N + d] = B.wd[q][k * N + i];
+  }
+}
+
+// Pair and wall candidate lists of the permuted scene (the single-scene list
+// builders with this scene's grid), then the scene's buffer parity flips.
+template <class H>
+__global__ __launch_bounds__(256) void kb_lists(BBufs<H> B) {
+  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
+  if (!any_due(B, step)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sid = due_scene(B, i, step);
+  if (sid < 0) return;
+  const BScene& S = B.scenes[sid];
+  SlotCtl& C = B.sctl[sid];
+  const int qn = __builtin_amdgcn_readfirstlane(C.q2);
+  const int cur = (step & 1) ^ qn;
+  ListArgs L;
+  L.pos = B.pos[cur];
+  L.pid = B.pid[qn];
+  L.cell_of = B.cell_of;
+  L.cell_tab = B.cell_tab;
+  L.nbr = B.nbr;
+  L.nnb = B.nnb;
+  L.ref = B.ref;
+  L.wl = B.wl;
+  L.nwl = B.nwl;
+  L.tri = B.tri;
+  L.bvh = nullptr;
+  L.tbox = B.tbox;
+  L.wc_off = B.wc_off;
+  L.wc_tri = B.wc_tri;
+  L.err = &C.err;
+  L.N = B.cap;
+  for (int a = 0; a < 3; ++a) L.size[a] = S.size[a];
+  L.cell_base = S.cell_base;
+  L.skin = S.skin;
+  L.nb_cap = B.nb_cap;
+  L.wall_cap = B.wall_cap;
+  L.ntri = S.ntri;
+  L.wall_table = 1;
+  const bool active = (i - S.seg0) < C.nact;
+  pairlist_one(L, i, active);
+  walllist_one(L, i, active);
+  // every later reader of q runs in a later kernel; this kernel reads q2 only
+  if (__lane_id() == 0) C.q = qn;
+  if (i == S.seg0) C.rebuilds += 1;
+}
+
+// Fused contact step of every running scene.
+template <class T, class H, class PRow, class WRow>
+__global__ __launch_bounds__(256, 4) void kb_step(BBufs<H> B) {
+  __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
+  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
+  const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int sid = __builtin_amdgcn_readfirstlane(B.wave_scene[__builtin_amdgcn_readfirstlane(s >> 6)]);
+  uint32_t err = 0;
+  bool refresh = false;
+  if (sid >= 0) {
+    const BScene& S = B.scenes[sid];
+    SlotCtl& C = B.sctl[sid];
+    if (step >= S.start_step && step < S.end_step) {
+      const int q = __builtin_amdgcn_readfirstlane(C.q);
+      const int cur = (step & 1) ^ q;
+      StepArgs<H> A;
+      A.pin = B.pos[cur];
+      A.vin = B.vel[cur];
+      A.win = B.angv[cur];
+      A.pout = B.pos[cur ^ 1];
+      A.vout = B.vel[cur ^ 1];
+      A.wout = B.angv[cur ^ 1];
+      A.mat = B.mat[q];
+      A.hkey = B.hkey[q];
+      A.hd = B.hd[q];
+      A.nh = B.nh[q];
+      A.wkey = B.wkey[q];
+      A.wd = B.wd[q];
+      A.nw = B.nw[q];
+      A.ref = B.ref;
+      A.nbr = B.nbr;
+      A.nnb = B.nnb;
+      A.wl = B.wl;
+      A.nwl = B.nwl;
+      A.tri = B.tri;
+      A.N = B.cap;
+      A.hist_cap = B.hist_cap;
+      A.whist_cap = B.whist_cap;
+      A.dt = S.dt;
+      A.mu = S.mu;
+      for (int a = 0; a < 3; ++a) {
+        A.g[a] = S.g[a];
+        A.bmin[a] = S.bmin[a];
+        A.bmax[a] = S.bmax[a];
+      }
+      A.threshSq = S.threshSq;
+      Rec<T> none{};
+      step_particle<T, H, false, PRow, WRow>(A, s, lds, none, err, refresh);
+      // a refresh at the scene's last step would only move its final state
+      if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {
+        C.fl[step & 1] = 1;
+        B.ctl[BC_ANY0 + (step & 1)] = 1;
+      }
+      if (err) atomicOr(&C.err, err);
+    }
+    // the flag the preceding rebuild consumed
+    if ((s >> 6) == (S.seg0 >> 6) && __lane_id() == 0) C.fl[rebuild_parity(step)] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.ctl[BC_ANY0 + rebuild_parity(step)] = 0;
+  if (last_block(&B.ctl[BC_DONE])) {
+    B.ctl[BC_DONE] = 0;
+    atomicExch(&B.ctl[BC_STEP], step + 1);
+  }
+}
+
+// State of one scene in its particles' own order (ids within the scene).
+template <class H>
+__global__ __launch_bounds__(256) void kb_download(BBufs<H> B, int sid, double* x, double* v, double* w,
+                                                   int* act) {
+  const BScene& S = B.scenes[sid];
+  const int i = S.seg0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S.seg0 + S.nseg) return;
+  const int step = B.ctl[BC_STEP];
+  const int t = step < S.end_step ? step : S.end_step;
+  const int q = B.sctl[sid].q;
+  const int cur = (t & 1) ^ q;
+  const int id = B.pid[q][i];
+  if (id < 0 || id >= S.n) return;
+  const double4 p = B.pos[cur][i], vv = B.vel[cur][i], ww = B.angv[cur][i];
+  x[id * 3] = p.x; x[id * 3 + 1] = p.y; x[id * 3 + 2] = p.z;
+  v[id * 3] = vv.x; v[id * 3 + 1] = vv.y; v[id * 3 + 2] = vv.z;
+  w[id * 3] = ww.x; w[id * 3 + 1] = ww.y; w[id * 3 + 2] = ww.z;
+  act[id] = p.w < 0. ? 0 : 1;
+}
+
+// Per-scene list/contact sums for statistics: active, pair candidates, wall
+// candidates, live history slots.
+template <class H>
+__global__ __launch_bounds__(256) void kb_counts(BBufs<H> B, int sid, unsigned long long* acc) {
+  const BScene& S = B.scenes[sid];
+  const int i = S.seg0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int step = B.ctl[BC_STEP];
+  const int t = step < S.end_step ? step : S.end_step;
+  const int q = B.sctl[sid].q;
+  const int cur = (t & 1) ^ q;
+  unsigned long long a = 0, b = 0, c = 0, d = 0;
+  if (i < S.seg0 + S.nseg && B.pos[cur][i].w >= 0.) {
+    a = 1;
+    b = (unsigned long long)B.nnb[i];
+    c = (unsigned long long)B.nwl[i];
+    d = (unsigned long long)(B.nh[q][i] + B.nw[q][i]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off);
+    b += __shfl_down(b, off);
+    c += __shfl_down(c, off);
+    d += __shfl_down(d, off);
+  }
+  if (__lane_id() == 0 && (a | b | c | d)) {
+    atomicAdd(acc + 0, a);
+    atomicAdd(acc + 1, b);
+    atomicAdd(acc + 2, c);
+    atomicAdd(acc + 3, d);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+constexpr int BPCAP_LDS = 16;
+constexpr int BWCAP_LDS = 8;
+
+template <class T, class H>
+void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st) {
+  if (grid <= 0) return;
+  const int nslots = B.nslots;
+  kb_bin<H><<<grid, 256, 0, st>>>(B);
+  kb_scan<H><<<nslots, 256, 0, st>>>(B);
+  kb_permute<H><<<grid, 256, 0, st>>>(B);
+  kb_lists<H><<<grid, 256, 0, st>>>(B);
+  if (B.hist_cap <= BPCAP_LDS && B.whist_cap <= BWCAP_LDS)
+    kb_step<T, H, HistLds<H, BPCAP_LDS>, HistLds<H, BWCAP_LDS>><<<grid, 256, 0, st>>>(B);
+  else
+    kb_step<T, H, HistScan<H>, HistScan<H>><<<grid, 256, 0, st>>>(B);
+}
+
+template <class H>
+void launch_batch_download(const BBufs<H>& B, int sid, int nseg, double* x, double* v, double* w, int* act,
+                           hipStream_t st) {
+  kb_download<H><<<(nseg + 255) / 256, 256, 0, st>>>(B, sid, x, v, w, act);
+}
+
+template <class H>
+void launch_batch_counts(const BBufs<H>& B, int sid, int nseg, unsigned long long* acc, hipStream_t st) {
+  kb_counts<H><<<(nseg + 255) / 256, 256, 0, st>>>(B, sid, acc);
+}
+
+template void launch_batch_step<double, double>(const BBufs<double>&, int, hipStream_t);
+template void launch_batch_step<float, float>(const BBufs<float>&, int, hipStream_t);
+template void launch_batch_download<double>(const BBufs<double>&, int, int, double*, double*, double*, int*,
+                                            hipStream_t);
+template void launch_batch_download<float>(const BBufs<float>&, int, int, double*, double*, double*, int*,
+                                           hipStream_t);
+template void launch_batch_counts<double>(const BBufs<double>&, int, int, unsigned long long*, hipStream_t);
+template void launch_batch_counts<float>(const BBufs<float>&, int, int, unsigned long long*, hipStream_t);
+
+}  // namespace gdem

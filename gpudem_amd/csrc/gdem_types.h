@@ -110,4 +110,86 @@ struct Bufs {
   int* ctl;
 };
 
+// ---------------------------------------------------------------------------
+// batched scenes
+// ---------------------------------------------------------------------------
+// Many independent scenes share one set of arrays.  Scene s owns a contiguous
+// particle segment [seg0, seg0 + nseg) whose length is a multiple of 64, so
+// every wave belongs to exactly one scene; a contiguous range of global cell
+// ids [cell_base, cell_base + ncell] (the extra entry ends the scene's wall
+// table); and a contiguous range of triangles.  Constants are written once at
+// admission.
+struct BScene {
+  double dt, mu;
+  double g[3];
+  double bmin[3], bmax[3];
+  double invd[3];
+  double skin, threshSq;
+  int size[3];
+  int cell_base, ncell;
+  int seg0, nseg;
+  int n;  // real particles (the rest of the segment is inactive padding)
+  int ntri, wall_table;
+  int start_step, end_step;  // the scene steps while start_step <= STEP < end_step
+};
+
+// Mutable per-scene words.  The current state buffer of a scene is
+// (STEP & 1) ^ q and its current material/history buffer is q: a rebuild
+// permutes the scene into the other buffers and flips q, leaving every other
+// scene untouched.
+struct SlotCtl {
+  int q;        // buffer parity
+  int q2;       // parity after the rebuild in progress
+  int fl[2];    // refresh requested by the step kernel at STEP parity 0 / 1
+  int inact;    // inactive particles counted by the binning pass
+  int nact;     // active particles after the last rebuild
+  unsigned int err;
+  int rebuilds;
+};
+
+enum BCtl : int {
+  BC_STEP = 0,  // batch steps completed
+  BC_ANY0 = 1,  // some scene requested a rebuild at STEP parity 0 / 1
+  BC_ANY1 = 2,
+  BC_DONE = 3,  // last-block counter of the step kernel
+  BC_WORDS = 8
+};
+
+template <class H>
+struct BBufs {
+  using H4 = typename Vec4<H>::type;
+  double4* pos[2];
+  double4* vel[2];
+  double4* angv[2];
+  H4* mat[2];
+  int* pid[2];  // id within the scene, -1 for padding
+  int* hkey[2];
+  H4* hd[2];
+  int* nh[2];
+  int* wkey[2];
+  H4* wd[2];
+  int* nw[2];
+  double4* ref;
+  int2* nbr;
+  int* nnb;
+  int* wl;
+  int* nwl;
+  int* bin_cell;   // global cell id of each slot at binning (-1 inactive)
+  int* bin_rank;   // rank within its cell (or among the inactive)
+  int* cell_of;    // global cell id after the permutation
+  int* ccount;     // per-cell counters, zero outside a rebuild
+  int2* cell_tab;  // per-cell (first, end) slots
+  const TriGeo* tri;
+  const double* tbox;
+  const int* wc_off;  // per global cell (plus one end entry per scene)
+  const int* wc_tri;  // global triangle ids
+  const int* wave_scene;  // scene of every wave of 64 slots, -1 if free
+  const BScene* scenes;
+  SlotCtl* sctl;
+  int* ctl;
+  int cap;  // slot stride of the slot-major arrays
+  int nb_cap, wall_cap, hist_cap, whist_cap;
+  int nslots;
+};
+
 }  // namespace gdem

@@ -17,8 +17,9 @@ LIB_PATH = os.path.join(_HERE, "libgdem.so")
 
 GDEM_FP64 = 0
 GDEM_MIXED = 1
+GDEM_E_BUSY = -7
 ERRORS = {-1: "GDEM_E_ARG", -2: "GDEM_E_HIP", -3: "GDEM_E_OOM", -4: "GDEM_E_SLOT_OVERFLOW",
-          -5: "GDEM_E_NAN", -6: "GDEM_E_STATE"}
+          -5: "GDEM_E_NAN", -6: "GDEM_E_STATE", -7: "GDEM_E_BUSY"}
 
 # exported symbols (checked by the CPU test suite against include/gdem.h)
 SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_material",
@@ -26,7 +27,9 @@ SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error"
            "gdem_scene_status", "gdem_scene_download", "gdem_scene_destroy", "gdem_scene_debug",
            "gdem_scene_step_record", "gdem_scene_timing", "gdem_scene_set_timing",
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
-           "gdem_copy_probe"]
+           "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
+           "gdem_batch_step", "gdem_batch_poll", "gdem_batch_download", "gdem_batch_scene_status",
+           "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing"]
 
 
 class GdemError(RuntimeError):
@@ -65,6 +68,21 @@ class Stats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("rebuilds", ctypes.c_int64), ("n_active", ctypes.c_int),
                 ("errors", ctypes.c_uint32), ("mean_nb", ctypes.c_double),
                 ("mean_wall", ctypes.c_double), ("mean_contacts", ctypes.c_double)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("max_scenes", ctypes.c_int), ("hbm_budget", ctypes.c_int64),
+                ("particle_capacity", ctypes.c_int64), ("cell_capacity", ctypes.c_int64),
+                ("tri_capacity", ctypes.c_int64), ("wall_capacity", ctypes.c_int64),
+                ("nb_cap", ctypes.c_int), ("wall_cap", ctypes.c_int), ("hist_cap", ctypes.c_int),
+                ("whist_cap", ctypes.c_int), ("graph_steps", ctypes.c_int)]
+
+
+class BatchSceneStats(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("start_step", ctypes.c_int64), ("end_step", ctypes.c_int64),
+                ("steps_done", ctypes.c_int64), ("rebuilds", ctypes.c_int), ("errors", ctypes.c_uint32),
+                ("n_active", ctypes.c_int), ("mean_nb", ctypes.c_double), ("mean_wall", ctypes.c_double),
+                ("mean_contacts", ctypes.c_double), ("bytes", ctypes.c_int64)]
 
 
 class DebugView(ctypes.Structure):
@@ -115,6 +133,19 @@ def lib():
         L.gdem_fp64_probe.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.gdem_mesh_precompute.argtypes = [P(MeshDesc), vp, vp, vp]
         L.gdem_copy_probe.argtypes = [vp, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
+        L.gdem_batch_create.argtypes = [vp, P(BatchParams), P(vp)]
+        L.gdem_batch_destroy.argtypes = [vp]
+        L.gdem_batch_destroy.restype = None
+        L.gdem_batch_admit.argtypes = [vp, P(ParticlesDesc), P(MeshDesc), P(Params), ctypes.c_int,
+                                       P(ctypes.c_int)]
+        L.gdem_batch_step.argtypes = [vp, ctypes.c_int]
+        L.gdem_batch_poll.argtypes = [vp, vp, ctypes.c_int, P(ctypes.c_int)]
+        L.gdem_batch_download.argtypes = [vp, ctypes.c_int, P(ParticlesOut)]
+        L.gdem_batch_scene_status.argtypes = [vp, ctypes.c_int, P(BatchSceneStats)]
+        L.gdem_batch_evict.argtypes = [vp, ctypes.c_int]
+        L.gdem_batch_info.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int), P(ctypes.c_int64),
+                                      P(ctypes.c_int64)]
+        L.gdem_batch_timing.argtypes = [vp, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -177,6 +208,35 @@ def bbox_rule(verts, inlet):
     if rc:
         raise GdemError(rc, "gdem_bbox_rule")
     return out
+
+
+def scene_inputs(x, r, m, k, etaconst, verts, params, v=None, angv=None, active=None, tris_per_file=None):
+    """C descriptors of one scene (particles, mesh, params) and the arrays they point into."""
+    x = _f64(x, (-1, 3))
+    n = x.shape[0]
+    arrs = dict(x=x, r=_f64(r).ravel(), m=_f64(m).ravel(), k=_f64(k).ravel(), e=_f64(etaconst).ravel(),
+                v=None if v is None else _f64(v, (-1, 3)),
+                w=None if angv is None else _f64(angv, (-1, 3)),
+                a=None if active is None else np.ascontiguousarray(active, dtype=np.int32))
+    pd = ParticlesDesc(n, _ptr(arrs["x"]), _ptr(arrs["v"]), _ptr(arrs["w"]), _ptr(arrs["r"]),
+                       _ptr(arrs["m"]), _ptr(arrs["k"]), _ptr(arrs["e"]), _ptr(arrs["a"]))
+    md = mesh_desc(verts if verts is not None else np.zeros((0, 9)), tris_per_file)
+    p = Params()
+    p.dt = params["dt"]
+    p.mu = params["mu"]
+    p.g = (ctypes.c_double * 3)(*params["g"])
+    p.bbox = (ctypes.c_double * 6)(*params["bbox"])
+    p.cell = params["cell"]
+    p.skin = params["skin"]
+    p.refresh = params["refresh"]
+    p.precision = params.get("precision", GDEM_FP64)
+    p.nb_cap = params.get("nb_cap", 0)
+    p.wall_cap = params.get("wall_cap", 0)
+    p.hist_cap = params.get("hist_cap", 0)
+    p.whist_cap = params.get("whist_cap", 0)
+    p.graph_steps = params.get("graph_steps", 0)
+    p.wall_broadphase = params.get("wall_broadphase", 0)
+    return pd, md, p, arrs
 
 
 class Context:
@@ -256,31 +316,8 @@ class Scene:
     def __init__(self, ctx, x, r, m, k, etaconst, verts, params, v=None, angv=None, active=None,
                  tris_per_file=None):
         self.ctx = ctx
-        x = _f64(x, (-1, 3))
-        self.n = x.shape[0]
-        arrs = dict(x=x, r=_f64(r).ravel(), m=_f64(m).ravel(), k=_f64(k).ravel(),
-                    e=_f64(etaconst).ravel(),
-                    v=None if v is None else _f64(v, (-1, 3)),
-                    w=None if angv is None else _f64(angv, (-1, 3)),
-                    a=None if active is None else np.ascontiguousarray(active, dtype=np.int32))
-        pd = ParticlesDesc(self.n, _ptr(arrs["x"]), _ptr(arrs["v"]), _ptr(arrs["w"]), _ptr(arrs["r"]),
-                           _ptr(arrs["m"]), _ptr(arrs["k"]), _ptr(arrs["e"]), _ptr(arrs["a"]))
-        md = mesh_desc(verts if verts is not None else np.zeros((0, 9)), tris_per_file)
-        p = Params()
-        p.dt = params["dt"]
-        p.mu = params["mu"]
-        p.g = (ctypes.c_double * 3)(*params["g"])
-        p.bbox = (ctypes.c_double * 6)(*params["bbox"])
-        p.cell = params["cell"]
-        p.skin = params["skin"]
-        p.refresh = params["refresh"]
-        p.precision = params.get("precision", GDEM_FP64)
-        p.nb_cap = params.get("nb_cap", 0)
-        p.wall_cap = params.get("wall_cap", 0)
-        p.hist_cap = params.get("hist_cap", 0)
-        p.whist_cap = params.get("whist_cap", 0)
-        p.graph_steps = params.get("graph_steps", 0)
-        p.wall_broadphase = params.get("wall_broadphase", 0)
+        pd, md, p, keep = scene_inputs(x, r, m, k, etaconst, verts, params, v, angv, active, tris_per_file)
+        self.n = pd.n
         self.caps = dict(nb=p.nb_cap or 48, wall=p.wall_cap or 24, hist=p.hist_cap or 16,
                          whist=p.whist_cap or 8)
         self.h = ctypes.c_void_p()
@@ -346,6 +383,98 @@ class Scene:
         if self.h:
             if self.ctx.h:
                 lib().gdem_scene_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent scenes stepped together (gdem_batch).
+
+    Capacities are fixed at creation (``particle_capacity`` or ``hbm_budget``
+    bytes); ``admit`` returns a scene id or None when no slot or capacity is
+    free (GDEM_E_BUSY); ``step`` advances every running scene; finished scenes
+    show up in ``poll`` until evicted.
+    """
+
+    def __init__(self, ctx, precision=GDEM_FP64, max_scenes=256, hbm_budget=0, particle_capacity=0,
+                 cell_capacity=0, tri_capacity=0, wall_capacity=0, nb_cap=0, wall_cap=0, hist_cap=0,
+                 whist_cap=0, graph_steps=0):
+        self.ctx = ctx
+        self.precision = precision
+        bp = BatchParams(precision, max_scenes, int(hbm_budget), int(particle_capacity), int(cell_capacity),
+                         int(tri_capacity), int(wall_capacity), nb_cap, wall_cap, hist_cap, whist_cap,
+                         graph_steps)
+        self.h = ctypes.c_void_p()
+        self.n_of = {}
+        ctx.check(lib().gdem_batch_create(ctx.h, ctypes.byref(bp), ctypes.byref(self.h)), "gdem_batch_create")
+        ctx._scenes.add(self)
+
+    def admit(self, x, r, m, k, etaconst, verts, params, nsteps, v=None, angv=None, active=None,
+              tris_per_file=None):
+        pd, md, p, keep = scene_inputs(x, r, m, k, etaconst, verts, params, v, angv, active, tris_per_file)
+        sid = ctypes.c_int(-1)
+        rc = lib().gdem_batch_admit(self.h, ctypes.byref(pd), ctypes.byref(md), ctypes.byref(p), int(nsteps),
+                                    ctypes.byref(sid))
+        if rc == GDEM_E_BUSY:
+            return None
+        self.ctx.check(rc, "gdem_batch_admit")
+        self.n_of[sid.value] = pd.n
+        return sid.value
+
+    def step(self, n=1):
+        self.ctx.check(lib().gdem_batch_step(self.h, int(n)), "gdem_batch_step")
+
+    def poll(self):
+        ids = (ctypes.c_int * 4096)()
+        n = ctypes.c_int()
+        self.ctx.check(lib().gdem_batch_poll(self.h, ids, 4096, ctypes.byref(n)), "gdem_batch_poll")
+        return [ids[i] for i in range(min(n.value, 4096))]
+
+    def download(self, sid):
+        n = self.n_of[sid]
+        x = np.empty((n, 3))
+        v = np.empty((n, 3))
+        w = np.empty((n, 3))
+        a = np.empty(n, np.int32)
+        out = ParticlesOut(_ptr(x), _ptr(v), _ptr(w), _ptr(a))
+        self.ctx.check(lib().gdem_batch_download(self.h, int(sid), ctypes.byref(out)), "gdem_batch_download")
+        return dict(x=x, v=v, angv=w, active=a)
+
+    def status(self, sid, check=True):
+        st = BatchSceneStats()
+        rc = lib().gdem_batch_scene_status(self.h, int(sid), ctypes.byref(st))
+        if check:
+            self.ctx.check(rc, "gdem_batch_scene_status")
+        return {f: getattr(st, f) for f, _ in BatchSceneStats._fields_}
+
+    def evict(self, sid):
+        self.ctx.check(lib().gdem_batch_evict(self.h, int(sid)), "gdem_batch_evict")
+        self.n_of.pop(sid, None)
+
+    def info(self):
+        step = ctypes.c_int64()
+        live = ctypes.c_int()
+        free = ctypes.c_int64()
+        nbytes = ctypes.c_int64()
+        self.ctx.check(lib().gdem_batch_info(self.h, ctypes.byref(step), ctypes.byref(live), ctypes.byref(free),
+                                             ctypes.byref(nbytes)), "gdem_batch_info")
+        return dict(step=step.value, live=live.value, free_particles=free.value, bytes=nbytes.value)
+
+    def timing(self):
+        """Device ms of the last step() call (waits for it)."""
+        ms = ctypes.c_double()
+        self.ctx.check(lib().gdem_batch_timing(self.h, ctypes.byref(ms)), "gdem_batch_timing")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                lib().gdem_batch_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

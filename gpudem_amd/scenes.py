@@ -223,3 +223,82 @@ def read_ascii_stl(paths):
                 i += 1
         counts.append(cnt)
     return np.array(tris, dtype=np.float64).reshape(-1, 9), counts
+
+
+def _seed(master, idx):
+    """Per-scene seed derived from (master seed, scene index)."""
+    return np.random.SeedSequence([int(master), int(idx)])
+
+
+def box_scene(seed, n, r0=0.001, r_lo=0.9, r_hi=1.1, phi=0.3, mu=0.5, cor=0.5, density=2500., young=1e7,
+              dt=5e-6, skin_factor=0.1, steps=5000, name="box"):
+    """n spheres, r ~ U[r_lo, r_hi] r0, at rest in a closed cube (12
+    triangles) whose volume makes the spheres a fraction `phi` of it; gravity
+    -z.  Spheres sit on jittered square layers filled from the floor, each
+    layer in the hollows of the one below (spacing 2.05 r_max, rise
+    spacing/sqrt(2)), so none overlap.  Verlet skin skin_factor * r_max,
+    refresh at skin/2, cell 2 r_max + skin."""
+    rng = np.random.default_rng(seed)
+    radius = rng.uniform(r_lo * r0, r_hi * r0, size=n)
+    rmax = r_hi * r0
+    vol = float(np.sum(4.0 / 3.0 * math.pi * radius ** 3))
+    side = (vol / phi) ** (1.0 / 3.0)
+    h = side / 2
+    clear = 1.03 * rmax
+    sp = 2.05 * rmax
+    span = side - 2 * clear
+    dz = sp / math.sqrt(2.0)
+    pts = []
+    layer = 0
+    while len(pts) < n:
+        z = -h + clear + layer * dz
+        if z > h - clear:
+            raise ValueError("box too small for the requested solid fraction")
+        off = 0.5 * sp if layer % 2 else 0.0
+        m = int((span - off) / sp) + 1
+        c = -h + clear + off + np.arange(m) * sp
+        Y, X = np.meshgrid(c, c, indexing="ij")
+        pts.extend(zip(X.ravel(), Y.ravel(), np.full(X.size, z)))
+        layer += 1
+    x = np.array(pts[:n], dtype=np.float64)
+    # jitter small enough that neither a diagonal neighbour nor a wall is reached
+    j = 0.9 * min(sp - 2 * rmax, 2 * (clear - rmax)) / (2 * math.sqrt(3.0))
+    x = x + rng.uniform(-j, j, size=x.shape)
+    verts = box_mesh((-h, -h, -h), (h, h, h))
+    bbox = bbox_rule(verts, [-h, h, -h, h, -h, h])
+    skin = skin_factor * rmax
+    params = dict(dt=dt, mu=mu, g=(0., 0., -9.81), bbox=bbox, cell=2 * rmax + skin, skin=skin,
+                  refresh=0.5 * skin, nb_cap=32, wall_cap=16, hist_cap=16, whist_cap=8)
+    return SceneSpec(name, x, radius, verts, params, density, young, cor, steps=steps)
+
+
+def batch_mix(seed=4, nscenes=256, nmin=1000, nmax=50000, steps=5000, **kw):
+    """The batched workload: independent box scenes, sphere counts log-uniform
+    over [nmin, nmax], friction ~ U[0.2, 0.8] and restitution ~ U[0.3, 0.9]
+    per scene, cubes sized to 30 % solid fraction."""
+    out = []
+    for i in range(nscenes):
+        rng = np.random.default_rng(_seed(seed, i))
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        mu = float(rng.uniform(0.2, 0.8))
+        cor = float(rng.uniform(0.3, 0.9))
+        out.append(box_scene(_seed(seed, 100000 + i), n, mu=mu, cor=cor, steps=steps, name=f"box{i}", **kw))
+    return out
+
+
+def served_stream(seed=5, nreq=100, rate=20.0, nmin=500, nmax=200000, smin=1000, smax=20000, **kw):
+    """The served workload: a Poisson stream (mean `rate` requests/s) of
+    drop-into-box scenes with sphere counts log-uniform over [nmin, nmax],
+    r ~ U[0.8, 1.2] r0 and U[smin, smax] steps each.  Returns a list of
+    (arrival_time_s, spec, nsteps)."""
+    rng = np.random.default_rng(_seed(seed, 0))
+    t = 0.0
+    out = []
+    for i in range(nreq):
+        t += float(rng.exponential(1.0 / rate))
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        nsteps = int(rng.integers(smin, smax + 1))
+        spec = box_scene(_seed(seed, 100000 + i), n, r_lo=0.8, r_hi=1.2, phi=0.2, steps=nsteps,
+                         name=f"req{i}", **kw)
+        out.append((t, spec, nsteps))
+    return out

@@ -25,6 +25,7 @@ extern "C" {
 #define GDEM_E_SLOT_OVERFLOW (-4) /* a neighbour / contact slot row overflowed */
 #define GDEM_E_NAN (-5)           /* non-finite particle state detected */
 #define GDEM_E_STATE (-6)         /* call not valid in the object's state */
+#define GDEM_E_BUSY (-7)          /* no free scene slot or capacity now: retry after an eviction */
 
 /* precision of the contact arithmetic */
 #define GDEM_FP64 0  /* plain FP64, bit-identical to the reference */
@@ -193,6 +194,61 @@ int gdem_scene_timing(gdem_scene* scene, double* total_ms, double* force_ms, int
 /* every > 0: time the whole call and the force kernel of every `every`-th
  * step (direct launches, no graph); 0 disables timing. */
 int gdem_scene_set_timing(gdem_scene* scene, int every);
+
+/* ---- batched and served scenes ------------------------------------------- */
+/* No counterpart in the reference, which steps one scene per process.  A
+ * batch holds many independent scenes in one set of device arrays sized at
+ * creation (an HBM budget); scenes are admitted and evicted while others keep
+ * running, and every gdem_batch_step advances all running scenes together in
+ * one launch per phase (captured in hipGraphs).  Each scene's trajectory is
+ * bit-identical to the same scene run alone through gdem_scene_*. */
+typedef struct gdem_batch gdem_batch;
+
+typedef struct {
+  int precision;              /* GDEM_FP64 or GDEM_MIXED, for every scene */
+  int max_scenes;             /* scene slots (0 = 256) */
+  int64_t hbm_budget;         /* device bytes for the batch (used when particle_capacity == 0) */
+  int64_t particle_capacity;  /* particle slots over all scenes (each scene rounds up to 64) */
+  int64_t cell_capacity;      /* grid cells over all scenes (0 = 3 x particles + 65536) */
+  int64_t tri_capacity;       /* triangles over all scenes (0 = particles / 16 + 4096) */
+  int64_t wall_capacity;      /* per-cell wall-table entries (0 = 4 x cells) */
+  int nb_cap, wall_cap, hist_cap, whist_cap; /* per-particle slots (0 = 48, 24, 16, 8) */
+  int graph_steps;            /* > 0: steps per captured hipGraph; 0: direct launches */
+} gdem_batch_params;
+
+typedef struct {
+  int n;                /* particles */
+  int64_t start_step;   /* batch step at admission */
+  int64_t end_step;     /* batch step at which the scene stops (start + nsteps) */
+  int64_t steps_done;
+  int rebuilds;
+  uint32_t errors;      /* GDEM_ERR_* bits */
+  int n_active;
+  double mean_nb, mean_wall, mean_contacts;
+  int64_t bytes;        /* device bytes reserved for the scene */
+} gdem_batch_scene_stats;
+
+int gdem_batch_create(gdem_ctx* ctx, const gdem_batch_params* params, gdem_batch** out);
+/* Destroy every batch before the context it was created on. */
+void gdem_batch_destroy(gdem_batch* batch);
+/* Admits a scene that will run nsteps batch steps from the next one.  Inputs
+ * as for gdem_scene_create (params.precision, capacities and graph_steps are
+ * the batch's).  GDEM_E_BUSY when no slot or capacity is free now. */
+int gdem_batch_admit(gdem_batch* batch, const gdem_particles_desc* particles, const gdem_mesh_desc* mesh,
+                     const gdem_params* params, int nsteps, int* scene_out);
+/* Enqueues nsteps batch steps (asynchronous on the context stream). */
+int gdem_batch_step(gdem_batch* batch, int nsteps);
+/* Scenes that have run all their steps and are not evicted yet. */
+int gdem_batch_poll(gdem_batch* batch, int* scenes, int max_scenes, int* n_out);
+int gdem_batch_download(gdem_batch* batch, int scene, gdem_particles_out* out);
+int gdem_batch_scene_status(gdem_batch* batch, int scene, gdem_batch_scene_stats* stats);
+/* Frees a scene's slot and ranges (finished or not). */
+int gdem_batch_evict(gdem_batch* batch, int scene);
+/* Batch step count, live scenes, free particle slots, device bytes held. */
+int gdem_batch_info(gdem_batch* batch, int64_t* step, int* live_scenes, int64_t* free_particles,
+                    int64_t* bytes);
+/* Device time (ms) of the last gdem_batch_step call; waits for it. */
+int gdem_batch_timing(gdem_batch* batch, double* ms);
 
 /* ---- primitives exposed for parity tests ---------------------------------- */
 /* Stable LSD radix sort of (key, iota) on the device: the order

@@ -1,0 +1,145 @@
+"""Batched and served scenes: every scene stepped inside a batch must end in
+exactly the state it reaches when stepped alone (gdem_scene, itself
+bit-identical to the oracle in FP64).  Results are compared bit for bit, in
+both precisions, for scenes admitted together, admitted while others run,
+and admitted into the ranges of evicted scenes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gpudem_amd import GDEM_FP64, GDEM_MIXED, Batch, Scene, engine, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CAPS = dict(nb_cap=32, wall_cap=16, hist_cap=16, whist_cap=8)
+
+
+def mk(seed, n, **kw):
+    return scenes.box_scene(seed, n, **kw)
+
+
+def admit(batch, spec, nsteps):
+    m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
+    return batch.admit(spec.x, spec.r, m, k, e, spec.verts, spec.params, nsteps, v=spec.v)
+
+
+def alone(ctx, spec, precision, nsteps):
+    m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
+    sc = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, dict(spec.params, precision=precision), v=spec.v)
+    sc.step(nsteps)
+    st = sc.status()
+    d = sc.download()
+    sc.close()
+    return d, st
+
+
+def assert_same(a, b, what):
+    for key in ("x", "v", "angv", "active"):
+        bad = np.where(np.any((a[key] != b[key]).reshape(len(a[key]), -1), axis=1))[0]
+        assert len(bad) == 0, f"{what}: {key} differs for {len(bad)} particles, first {bad[:5]}"
+
+
+def mixed_specs():
+    # uneven sizes, per-scene friction, restitution and time step
+    return [mk(1, 300, mu=0.2, cor=0.3), mk(2, 1500, mu=0.8, cor=0.9, dt=4e-6), mk(3, 2600, mu=0.5, cor=0.6),
+            mk(4, 700, mu=0.35, cor=0.45, dt=6e-6), mk(5, 4100, mu=0.65, cor=0.75), mk(6, 130, mu=0.5, cor=0.5)]
+
+
+@pytest.mark.parametrize("precision,graph", [(GDEM_FP64, 0), (GDEM_MIXED, 100)])
+def test_batch_scenes_match_alone(ctx, precision, graph):
+    specs = mixed_specs()
+    nsteps = 1500
+    say(f"batch of {len(specs)} box scenes, precision {precision}, graph_steps {graph}, {nsteps} steps")
+    b = Batch(ctx, precision=precision, max_scenes=16, particle_capacity=16384, graph_steps=graph, **CAPS)
+    ids = [admit(b, s, nsteps) for s in specs]
+    assert all(i is not None for i in ids)
+    b.step(nsteps)
+    assert sorted(b.poll()) == sorted(ids)
+    for sid, spec in zip(ids, specs):
+        st = b.status(sid)
+        assert st["errors"] == 0 and st["steps_done"] == nsteps and st["rebuilds"] > 0
+        d_alone, st_alone = alone(ctx, spec, precision, nsteps)
+        say(f"  scene {spec.name} n={spec.n} rebuilds {st['rebuilds']} (alone {st_alone['rebuilds']}) "
+            f"contacts {st['mean_contacts']:.2f}")
+        assert_same(b.download(sid), d_alone, spec.name)
+    b.close()
+
+
+def test_batch_fp64_matches_oracle(ctx):
+    specs = [mk(7, 400, mu=0.3, cor=0.4), mk(8, 900, mu=0.7, cor=0.8, dt=4e-6)]
+    nsteps = 2000
+    say(f"batch vs oracle, 2 scenes, {nsteps} steps")
+    b = Batch(ctx, precision=GDEM_FP64, max_scenes=4, particle_capacity=4096, **CAPS)
+    ids = [admit(b, s, nsteps) for s in specs]
+    b.step(nsteps)
+    for sid, spec in zip(ids, specs):
+        orc = oracle.Sim(spec.x, spec.r, oracle.Mesh(spec.verts), spec.oracle_params())
+        orc.step(nsteps)
+        ref = dict(x=orc.get("x").reshape(-1, 3), v=orc.get("v").reshape(-1, 3),
+                   angv=orc.get("angv").reshape(-1, 3), active=orc.get("isActive"))
+        assert_same(b.download(sid), ref, spec.name)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_continuous_admission_and_eviction(ctx, precision):
+    say(f"continuous batching, precision {precision}")
+    b = Batch(ctx, precision=precision, max_scenes=3, particle_capacity=8192, graph_steps=50, **CAPS)
+    a, bb, c, d = mk(11, 3000), mk(12, 2000, mu=0.7), mk(13, 4000), mk(14, 2500, cor=0.8)
+    ia = admit(b, a, 800)
+    b.step(300)
+    ib = admit(b, bb, 600)  # joins while A runs
+    b.step(500)
+    assert b.poll() == [ia]
+    assert_same(b.download(ia), alone(ctx, a, precision, 800)[0], "A")
+    b.evict(ia)
+    assert admit(b, c, 500) is None  # 4032 slots do not fit the fragments left (3008 + 3136)
+    id_ = admit(b, d, 700)  # fits where A was
+    assert id_ is not None
+    e = mk(15, 500)
+    ie = admit(b, e, 100)
+    assert admit(b, mk(16, 300), 10) is None  # all three scene slots taken
+    b.step(100)
+    assert sorted(b.poll()) == sorted([ib, ie])
+    assert_same(b.download(ie), alone(ctx, e, precision, 100)[0], "E")
+    b.step(600)
+    assert sorted(b.poll()) == sorted([ib, ie, id_])
+    assert_same(b.download(ib), alone(ctx, bb, precision, 600)[0], "B")
+    assert_same(b.download(id_), alone(ctx, d, precision, 700)[0], "D")
+    for s in (ib, ie, id_):
+        assert b.status(s)["errors"] == 0
+        b.evict(s)
+    assert b.info()["live"] == 0
+    b.close()
+
+
+def test_skin_and_cell_do_not_change_results(ctx):
+    # lists are supersets of the contacts whenever refresh <= skin/2, so the
+    # tuner's skin/cell choice never changes a trajectory
+    base = mk(21, 2000, mu=0.4, cor=0.5)
+    nsteps = 1200
+    say("skin invariance")
+    b = Batch(ctx, precision=GDEM_FP64, max_scenes=4, particle_capacity=8192, **CAPS)
+    ids = []
+    for f, cellf in ((0.1, 1.0), (0.3, 1.0), (0.3, 1.5)):
+        spec = mk(21, 2000, mu=0.4, cor=0.5, skin_factor=f)
+        spec.params["cell"] *= cellf
+        ids.append(admit(b, spec, nsteps))
+    b.step(nsteps)
+    ref = b.download(ids[0])
+    reb = [b.status(i)["rebuilds"] for i in ids]
+    say(f"  rebuilds per skin choice {reb}")
+    assert reb[0] > reb[1]
+    for i in ids[1:]:
+        assert_same(b.download(i), ref, f"scene {i}")
+    assert_same(ref, alone(ctx, base, GDEM_FP64, nsteps)[0], "alone")
+    b.close()
