@@ -106,6 +106,80 @@ def summarize(run, steps, mixed):
                 rebuilds=s1["rebuilds"] - run["stats0"]["rebuilds"])
 
 
+CAPS = dict(nb_cap=32, wall_cap=16, hist_cap=16, whist_cap=8)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_batched(ctx, specs, precision, tuned, steps):
+    """All scenes admitted together and stepped `steps` timed batch steps (one
+    launch per phase, hipGraph-captured).  Every scene ends at the same batch
+    step, so its completion latency is the batch's run time."""
+    from gpudem_amd import Batch, autotune, engine
+    total = sum(s.n for s in specs)
+    b = Batch(ctx, precision=precision, max_scenes=len(specs), particle_capacity=total + 64 * len(specs),
+              graph_steps=tuned["graph_steps"], **CAPS)
+    ids = []
+    # one extra graph's worth of steps runs first, untimed: it captures and
+    # instantiates the hipGraph; the timed window is `steps` of every scene
+    lead = tuned["graph_steps"]
+    t_a = time.perf_counter()
+    for s in specs:
+        m, k, e = engine.material(s.r, s.density, s.young, s.cor)
+        ids.append(b.admit(s.x, s.r, m, k, e, s.verts,
+                           autotune.with_tuning(s, tuned["skin_factor"], tuned["cell_factor"]), lead + steps,
+                           v=s.v))
+    ctx.sync()
+    admit_s = time.perf_counter() - t_a
+    b.step(lead)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    ms = b.timing()
+    wall = time.perf_counter() - t0
+    errors, rebuilds, contacts = 0, 0, 0.0
+    for sid, s in zip(ids, specs):
+        st = b.status(sid, check=False)
+        errors |= st["errors"]
+        rebuilds += st["rebuilds"]
+        contacts += st["mean_contacts"] * st["n_active"]
+    info = b.info()
+    b.close()
+    return dict(value=total * steps / (ms / 1e3), ms_per_step=ms / steps, scenes=len(specs), particles=total,
+                steps=steps, scene_latency_p50_s=wall, scene_latency_p99_s=wall, admit_s=admit_s,
+                errors=errors, rebuilds_per_scene=rebuilds / len(specs), mean_contacts=contacts / total,
+                batch_bytes=info["bytes"])
+
+
+def run_plain(ctx, specs, precision, steps):
+    """The single-scene path on the same scenes, one after another."""
+    from gpudem_amd import Scene, engine
+    work, secs = 0.0, 0.0
+    for s in specs:
+        m, k, e = engine.material(s.r, s.density, s.young, s.cor)
+        sc = Scene(ctx, s.x, s.r, m, k, e, s.verts, dict(s.params, precision=precision, **CAPS), v=s.v)
+        ctx.sync()
+        t0 = time.perf_counter()
+        sc.step(steps)
+        ctx.sync()
+        secs += time.perf_counter() - t0
+        work += s.n * steps
+        sc.close()
+    return dict(value=work / secs, scenes=len(specs), steps=steps, seconds=secs)
+
+
+def run_served(ctx, precision, nreq, budget, seed=5):
+    from gpudem_amd import scenes, serve
+    stream = scenes.served_stream(seed=seed, nreq=nreq)
+    srv = serve.Server(ctx, precision, hbm_budget=budget, keep_results=False, **CAPS)
+    done = srv.run(stream)
+    out = serve.summarize(done, srv.makespan)
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +196,16 @@ def main():
                     help="0: per-cell table when it fits, 1: BVH traversal, 2: per-cell table")
     ap.add_argument("--sample-every", type=int, default=10,
                     help="bracket the force kernel of every n-th timed step with HIP events")
+    ap.add_argument("--batch-steps", type=int, default=5000, help="batch steps of the 256-scene mix")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=300,
+                    help="steps per scene of the plain single-scene comparison")
+    ap.add_argument("--plain-sample", type=int, default=32, help="scenes of the mix timed on the plain path")
+    ap.add_argument("--served-requests", type=int, default=60)
+    ap.add_argument("--served-budget-gb", type=float, default=8.0)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (skin 0.1, cell 1.0, graph 100)")
     ap.add_argument("--save-state", help="write the mixed run's final state to this .npz")
     ap.add_argument("--load-state", help="start both runs from this .npz instead of the lattice")
     args = ap.parse_args()
@@ -191,6 +275,38 @@ def main():
         out["fp64_roofline"] = {"achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": d["achieved"] / HBM_PEAK_GBS, "bytes_per_particle_step": d["B"],
                                 "avg_launch_us": d["force_avg_s"] * 1e6, "kernel": "k_step<double,double>"}
+    if not args.no_batch:
+        from gpudem_amd import GDEM_FP64 as F64, autotune
+        mix = scenes.batch_mix(nscenes=args.batch_scenes, steps=args.batch_steps)
+        log(f"# batch mix: {len(mix)} scenes, {sum(s.n for s in mix)} spheres")
+        if args.no_tune:
+            tuned = dict(skin_factor=0.1, cell_factor=1.0, graph_steps=100)
+        else:
+            tuned, _ = autotune.tune_batch(ctx, mix, GDEM_MIXED, log=log)
+        bm = run_batched(ctx, mix, GDEM_MIXED, tuned, args.batch_steps)
+        log(f"# batched mixed: {bm['value']:.3e} particle-steps/s, {bm['ms_per_step']:.3f} ms/step")
+        bf = run_batched(ctx, mix, F64, tuned, args.batch_steps)
+        log(f"# batched fp64: {bf['value']:.3e} particle-steps/s")
+        sample = mix[::max(1, len(mix) // args.plain_sample)][:args.plain_sample]
+        pl = run_plain(ctx, sample, F64, args.plain_steps)
+        log(f"# plain fp64 (single-scene path, {len(sample)} scenes one at a time): {pl['value']:.3e}")
+        out["batched"] = {"workload": "box_mix_256_logU1e3-5e4", "value": bm["value"],
+                          "unit": "particle-steps/s", "dtype": "f32 forces / f64 integration",
+                          "fp64_value": bf["value"], "plain_fp64_value": pl["value"],
+                          "speedup_vs_plain_fp64": bm["value"] / pl["value"],
+                          "scene_latency_p50_s": bm["scene_latency_p50_s"],
+                          "scene_latency_p99_s": bm["scene_latency_p99_s"], "tuned": tuned,
+                          "mixed": bm, "fp64": bf, "plain_fp64": pl}
+    if not args.no_served:
+        budget = int(args.served_budget_gb * (1 << 30))
+        sv = {}
+        for name, prec in (("mixed", GDEM_MIXED), ("fp64", GDEM_FP64)):
+            sv[name] = run_served(ctx, prec, args.served_requests, budget)
+            log(f"# served {name}: {sv[name]['value']:.3e} particle-steps/s, p50 {sv[name]['p50_s']:.2f} s, "
+                f"p99 {sv[name]['p99_s']:.2f} s")
+        out["served"] = {"workload": f"poisson20_boxes_logU500-2e5_{args.served_requests}req",
+                         "hbm_budget_bytes": budget, "mixed": sv["mixed"], "fp64": sv["fp64"],
+                         "p50_s": sv["mixed"]["p50_s"], "p99_s": sv["mixed"]["p99_s"]}
     ctx.close()
     if not args.no_cpu:
         cpu = run_cpu(spec, mixed["state"], args.cpu_steps, 2)

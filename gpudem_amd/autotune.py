@@ -1,0 +1,74 @@
+"""Autotuner of the batched engine's free parameters.
+
+The lists are supersets of the contacts whenever refresh <= skin / 2 and
+cell >= 2 r_max + skin, so the Verlet skin, the cell size and the number of
+steps per captured hipGraph change only the speed, never a trajectory (the
+GPU tests check this bit for bit).  The tuner times each candidate on a
+sample of the real workload -- a few warm-up steps, then a timed window --
+and keeps the fastest.  The workgroup size is fixed at 256 (the step kernel's
+LDS history columns and the XCD tile mapping are laid out for it).
+"""
+import itertools
+import time
+
+from . import engine
+from .engine import Batch
+
+SKIN_FACTORS = (0.05, 0.1, 0.2, 0.4)
+CELL_FACTORS = (1.0, 1.25)
+GRAPH_STEPS = (50, 200)
+
+
+def with_tuning(spec, skin_factor, cell_factor):
+    """The scene's params with skin = f r_max, refresh = skin/2 and
+    cell = cell_factor (2 r_max + skin)."""
+    rmax = float(spec.r.max())
+    skin = skin_factor * rmax
+    return dict(spec.params, skin=skin, refresh=0.5 * skin, cell=cell_factor * (2 * rmax + skin))
+
+
+def time_batch(ctx, specs, precision, skin_factor, cell_factor, graph_steps, warm, steps, caps):
+    # whole graphs only, and the first one (captured and instantiated on the
+    # host) inside the warm-up
+    g = max(1, graph_steps)
+    warm = max(warm, g) // g * g + (g if max(warm, g) % g else 0)
+    steps = max(steps, g) // g * g
+    total = sum(s.n for s in specs)
+    b = Batch(ctx, precision=precision, max_scenes=len(specs), particle_capacity=total + 64 * len(specs),
+              graph_steps=graph_steps, **caps)
+    try:
+        for s in specs:
+            m, k, e = engine.material(s.r, s.density, s.young, s.cor)
+            sid = b.admit(s.x, s.r, m, k, e, s.verts, with_tuning(s, skin_factor, cell_factor), warm + steps,
+                          v=s.v)
+            if sid is None:
+                raise RuntimeError("tuning batch does not fit")
+        b.step(warm)
+        ctx.sync()
+        b.step(steps)
+        ms = b.timing()
+    finally:
+        b.close()
+    return ms / steps, total * steps / (ms / 1e3)
+
+
+def tune_batch(ctx, specs, precision, sample=16, warm=100, steps=300, caps=None, candidates=None, log=None):
+    """Times every (skin factor, cell factor, graph steps) candidate on every
+    k-th scene of `specs` (about `sample` scenes).  Returns (best, table) where
+    best = dict(skin_factor, cell_factor, graph_steps) and table lists
+    (candidate, ms_per_step, particle_steps_per_s)."""
+    caps = caps or dict(nb_cap=32, wall_cap=16, hist_cap=16, whist_cap=8)
+    stride = max(1, len(specs) // sample)
+    probe = specs[::stride][:sample]
+    cands = candidates or list(itertools.product(SKIN_FACTORS, CELL_FACTORS, GRAPH_STEPS))
+    table = []
+    t0 = time.perf_counter()
+    for sf, cf, g in cands:
+        ms, rate = time_batch(ctx, probe, precision, sf, cf, g, warm, steps, caps)
+        table.append((dict(skin_factor=sf, cell_factor=cf, graph_steps=g), ms, rate))
+        if log:
+            log(f"# tune skin {sf} cell {cf} graph {g}: {ms:.3f} ms/step, {rate:.3e} particle-steps/s")
+    best = max(table, key=lambda t: t[2])[0]
+    if log:
+        log(f"# tuned in {time.perf_counter() - t0:.1f} s: {best}")
+    return best, table

@@ -1,0 +1,123 @@
+"""Serving: a request loop over one ``Batch`` (continuous batching).
+
+Requests arrive on a wall-clock schedule.  Each loop iteration enqueues a
+chunk of batch steps, and while the device runs them, admits the requests
+that have arrived, in arrival order, as scene slots and capacity (the
+batch's HBM budget) allow.  The head of the queue waits when it does not fit;
+first-fit placement reuses the ranges of evicted scenes.  After the chunk, finished
+scenes are downloaded and evicted.  Every admitted scene keeps its state on
+the device and steps together with all others in the batched kernels, so a
+request that arrives while others run overlaps with them on the GPU.
+"""
+import collections
+import time
+
+import numpy as np
+
+from . import engine
+from .engine import Batch
+
+
+class Request:
+    __slots__ = ("rid", "arrival", "spec", "nsteps", "sid", "admitted", "done", "result", "stats")
+
+    def __init__(self, rid, arrival, spec, nsteps):
+        self.rid = rid
+        self.arrival = arrival
+        self.spec = spec
+        self.nsteps = nsteps
+        self.sid = None
+        self.admitted = None
+        self.done = None
+        self.result = None
+        self.stats = None
+
+
+class Server:
+    """Continuous-batching server of DEM scenes on one device.
+
+    chunk: batch steps per loop iteration (the scheduling quantum);
+    keep_results: download every finished scene's final state.
+    """
+
+    def __init__(self, ctx, precision, hbm_budget=8 << 30, max_scenes=256, chunk=200, graph_steps=100,
+                 nb_cap=32, wall_cap=16, hist_cap=16, whist_cap=8, keep_results=True):
+        self.ctx = ctx
+        self.chunk = chunk
+        self.keep_results = keep_results
+        self.batch = Batch(ctx, precision=precision, max_scenes=max_scenes, hbm_budget=hbm_budget,
+                           nb_cap=nb_cap, wall_cap=wall_cap, hist_cap=hist_cap, whist_cap=whist_cap,
+                           graph_steps=graph_steps)
+
+    def _admit(self, req, now):
+        s = req.spec
+        m, k, e = engine.material(s.r, s.density, s.young, s.cor)
+        sid = self.batch.admit(s.x, s.r, m, k, e, s.verts, s.params, req.nsteps, v=s.v)
+        if sid is None:
+            return False
+        req.sid = sid
+        req.admitted = now
+        return True
+
+    def run(self, stream):
+        """stream: iterable of (arrival_s, spec, nsteps).  Returns the list of
+        finished Requests (arrival, admitted and done times in seconds since
+        the start of the run)."""
+        pending = collections.deque(Request(i, a, s, n) for i, (a, s, n) in
+                                    enumerate(sorted(stream, key=lambda t: t[0])))
+        queue = collections.deque()
+        running = {}
+        finished = []
+        end_of = {}
+        t0 = time.perf_counter()
+        b = self.batch
+        while pending or queue or running:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            if running:
+                # run up to the earliest completion (at least a few steps)
+                step = b.info()["step"]
+                k = min(self.chunk, max(1, min(end_of[sid] - step for sid in running)))
+                b.step(k)
+            # admissions overlap the chunk on the device; they start after it
+            while queue:
+                req = queue[0]
+                if not self._admit(req, time.perf_counter() - t0):
+                    break
+                queue.popleft()
+                running[req.sid] = req
+                end_of[req.sid] = b.info()["step"] + req.nsteps
+            if not running:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                continue
+            self.ctx.sync()
+            done_now = time.perf_counter() - t0
+            for sid in b.poll():
+                req = running.pop(sid)
+                end_of.pop(sid)
+                req.stats = b.status(sid, check=False)
+                if self.keep_results:
+                    req.result = b.download(sid)
+                req.done = done_now
+                b.evict(sid)
+                finished.append(req)
+        self.makespan = time.perf_counter() - t0
+        return finished
+
+    def close(self):
+        self.batch.close()
+
+
+def summarize(finished, makespan):
+    """Throughput and latency percentiles of a finished run."""
+    lat = np.array([r.done - r.arrival for r in finished])
+    wait = np.array([r.admitted - r.arrival for r in finished])
+    work = float(sum(r.spec.n * r.nsteps for r in finished))
+    return dict(requests=len(finished), particle_steps=work, makespan_s=makespan,
+                value=work / makespan if makespan > 0 else 0.0,
+                p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), queue_wait_p50_s=float(np.percentile(wait, 50)),
+                queue_wait_p99_s=float(np.percentile(wait, 99)),
+                errors=int(sum(1 for r in finished if r.stats and r.stats["errors"])))

@@ -143,3 +143,37 @@ def test_skin_and_cell_do_not_change_results(ctx):
         assert_same(b.download(i), ref, f"scene {i}")
     assert_same(ref, alone(ctx, base, GDEM_FP64, nsteps)[0], "alone")
     b.close()
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_served_results_equal_alone(ctx, precision):
+    # a short Poisson-like stream through the continuous-batching server: every
+    # request's result equals the same scene stepped alone
+    from gpudem_amd import serve
+    rng = np.random.default_rng(3)
+    stream, t = [], 0.0
+    for i in range(6):
+        t += float(rng.exponential(0.05))
+        n = int(rng.integers(200, 3000))
+        nsteps = int(rng.integers(200, 900))
+        stream.append((t, mk(40 + i, n, r_lo=0.8, r_hi=1.2, phi=0.2, mu=float(rng.uniform(0.2, 0.8))), nsteps))
+    say(f"served stream of {len(stream)} requests, precision {precision}")
+    srv = serve.Server(ctx, precision, hbm_budget=64 << 20, max_scenes=4, chunk=64, graph_steps=32, **CAPS)
+    done = srv.run(stream)
+    srv.close()
+    assert len(done) == len(stream)
+    summ = serve.summarize(done, srv.makespan)
+    say(f"  p50 {summ['p50_s']:.3f} s p99 {summ['p99_s']:.3f} s, {summ['value']:.3e} particle-steps/s")
+    for req in done:
+        assert req.stats["errors"] == 0
+        assert req.stats["steps_done"] == req.nsteps
+        assert_same(req.result, alone(ctx, req.spec, precision, req.nsteps)[0], f"request {req.rid}")
+
+
+def test_autotuner_returns_a_timed_candidate(ctx):
+    from gpudem_amd import autotune
+    specs = [mk(50 + i, 500 + 300 * i) for i in range(4)]
+    best, table = autotune.tune_batch(ctx, specs, GDEM_MIXED, sample=4, warm=20, steps=40,
+                                      candidates=[(0.1, 1.0, 20), (0.3, 1.25, 40)], log=say)
+    assert len(table) == 2 and all(t[1] > 0 for t in table)
+    assert best in [t[0] for t in table]
