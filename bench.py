@@ -124,7 +124,7 @@ def run_batched(ctx, specs, precision, tuned, steps):
     ids = []
     # one extra graph's worth of steps runs first, untimed: it captures and
     # instantiates the hipGraph; the timed window is `steps` of every scene
-    lead = tuned["graph_steps"]
+    lead = max(1, tuned["graph_steps"])
     t_a = time.perf_counter()
     for s in specs:
         m, k, e = engine.material(s.r, s.density, s.young, s.cor)
@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (skin 0.1, cell 1.0, graph 100)")
+    ap.add_argument("--batch-graph-steps", type=int, default=None,
+                    help="override the tuned steps per hipGraph of the batched run (0: direct launches)")
     ap.add_argument("--save-state", help="write the mixed run's final state to this .npz")
     ap.add_argument("--load-state", help="start both runs from this .npz instead of the lattice")
     args = ap.parse_args()
@@ -283,6 +285,8 @@ def main():
             tuned = dict(skin_factor=0.1, cell_factor=1.0, graph_steps=100)
         else:
             tuned, _ = autotune.tune_batch(ctx, mix, GDEM_MIXED, log=log)
+        if args.batch_graph_steps is not None:
+            tuned = dict(tuned, graph_steps=args.batch_graph_steps)
         bm = run_batched(ctx, mix, GDEM_MIXED, tuned, args.batch_steps)
         log(f"# batched mixed: {bm['value']:.3e} particle-steps/s, {bm['ms_per_step']:.3f} ms/step")
         bf = run_batched(ctx, mix, F64, tuned, args.batch_steps)

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void kb_lists(BBufs<H> B) {
 
 // Fused contact step of every running scene.
 template <class T, class H, class PRow, class WRow>
-__global__ __launch_bounds__(256, 4) void kb_step(BBufs<H> B) {
+__global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
   __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256, 4) void kb_step(BBufs<H> B) {
       }
       A.threshSq = S.threshSq;
       Rec<T> none{};
-      step_particle<T, H, false, PRow, WRow>(A, s, lds, none, err, refresh);
+      step_particle<T, H, false, PRow, WRow>(A, s, true, lds, none, err, refresh);
       // a refresh at the scene's last step would only move its final state
       if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {
         C.fl[step & 1] = 1;

@@ -205,17 +205,39 @@ struct HistLds {
   int s, N, cap;
   int n;           // live slots
   uint64_t marks;  // slots in contact this step
-  int* k;          // this thread's LDS column: k[q * blockDim.x]
-  __device__ __forceinline__ void load(int nh) {
+  int* k;          // this thread's LDS column: k[q * 256]
+  int* k0;         // thread 0's column; thread t's column is k0 + t
+  // ok: this lane's particle is active (other lanes load nothing of theirs)
+  __device__ __forceinline__ void load(int nh, bool ok) {
     n = nh;
     marks = 0;
+    // Every key load is issued before the first LDS store, so one memory
+    // round trip serves the whole row.  The wave loads as many slots as its
+    // longest row (never more than the capacity); lanes without a particle
+    // read slot row 0 of particle 0 instead, so every address is inside the
+    // [cap][N] array.  Slots past a lane's own row are not stored.
+    int wmax = ok ? nh : 0;
+    for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
+    wmax = min(__builtin_amdgcn_readfirstlane(wmax), min(cap, CAP));
+    const int* src = gkey + (ok ? s : 0);
+    int kk[CAP];
+#pragma unroll
+    for (int q = 0; q < CAP; ++q) kk[q] = q < wmax ? ld_nt(src + (size_t)q * N) : 0;
 #pragma unroll
     for (int q = 0; q < CAP; ++q)
-      if (q < nh) k[q * 256] = ld_nt(gkey + (size_t)q * N + s);
+      if (q < nh) k[q * 256] = kk[q];
   }
   __device__ __forceinline__ int find(int key) const {
     for (int q = 0; q < n; ++q)
       if (k[q * 256] == key) return q;
+    return -1;
+  }
+  // slot of `key` among the first n_o slots of the particle held by thread
+  // col (slot index so)
+  __device__ __forceinline__ int find_in(int col, int so, int n_o, int key) const {
+    const int* c = k0 + col;
+    for (int q = 0; q < n_o; ++q)
+      if (c[q * 256] == key) return q;
     return -1;
   }
   __device__ __forceinline__ void remember(int key) { k[n * 256] = key; }
@@ -229,13 +251,18 @@ struct HistScan {
   int s, N, cap;
   int n;
   uint64_t marks;
-  __device__ __forceinline__ void load(int nh) {
+  __device__ __forceinline__ void load(int nh, bool) {
     n = nh;
     marks = 0;
   }
   __device__ __forceinline__ int find(int key) const {
     for (int q = 0; q < n; ++q)
       if (gkey[(size_t)q * N + s] == key) return q;
+    return -1;
+  }
+  __device__ __forceinline__ int find_in(int, int so, int n_o, int key) const {
+    for (int q = 0; q < n_o; ++q)
+      if (gkey[(size_t)q * N + so] == key) return q;
     return -1;
   }
   __device__ __forceinline__ void remember(int) {}
@@ -245,23 +272,13 @@ struct HistScan {
 // device_dem.cu:622-740; the wall twin :403-499 is identical with the wall
 // history).  The slot of `key` is found or appended (d_calc_tangential_force
 // :630-653); T is the force type and FP64 keeps the reference's operations.
-template <class T, class Row>
-__device__ __forceinline__ V3<T> tangential_at(Row& row, uint32_t& err, uint32_t ovf_bit, int key, int slot,
-                                               V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu,
-                                               T dt) {
-  using H4 = typename Row::H4;
-  if (slot < 0) {
-    // new contact: appended in list order (old is zero)
-    if (row.n < row.cap) {
-      slot = row.n;
-      st_nt(key, row.gkey + (size_t)slot * row.N + row.s);
-      row.remember(key);
-      ++row.n;
-    } else {
-      err |= ovf_bit;
-    }
-  }
-  if (slot >= 0) row.marks |= (uint64_t)1 << slot;
+// The spring-dashpot arithmetic of d_calc_tangential_force
+// (device_dem.cu:654-737): the displacement `old` advanced by vt dt and
+// projected onto the tangent plane, the Coulomb-capped spring force, and the
+// displacement to remember (returned in dn).
+template <class T>
+__device__ __forceinline__ V3<T> tangential_force(V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu, T dt,
+                                                  V3<T>& dn_out) {
   V3<T> dn{old.x + vt.x * dt, old.y + vt.y * dt, old.z + vt.z * dt};
   const T dd = dot(dn, n);
   dn = vsub(dn, scal(dd, n));
@@ -282,14 +299,44 @@ __device__ __forceinline__ V3<T> tangential_at(Row& row, uint32_t& err, uint32_t
     ft.y -= eta * vt.y;
     ft.z -= eta * vt.z;
   }
-  if (slot >= 0) {
-    H4 o;
-    o.x = (decltype(o.x))dn.x;
-    o.y = (decltype(o.y))dn.y;
-    o.z = (decltype(o.z))dn.z;
-    o.w = 0;
-    st_nt(o, row.gd + (size_t)slot * row.N + row.s);
+  dn_out = dn;
+  return ft;
+}
+
+template <class H, class T>
+__device__ __forceinline__ void store_delta(typename Vec4<H>::type* p, V3<T> dn) {
+  typename Vec4<H>::type o;
+  o.x = (decltype(o.x))dn.x;
+  o.y = (decltype(o.y))dn.y;
+  o.z = (decltype(o.z))dn.z;
+  o.w = 0;
+  st_nt(o, p);
+}
+
+// Tangential spring with Coulomb cap and history (d_calc_tangential_force,
+// device_dem.cu:622-740; the wall twin :403-499 is identical with the wall
+// history).  The slot of `key` is found or appended (d_calc_tangential_force
+// :630-653); T is the force type and FP64 keeps the reference's operations.
+template <class T, class Row>
+__device__ __forceinline__ V3<T> tangential_at(Row& row, uint32_t& err, uint32_t ovf_bit, int key, int slot,
+                                               V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu,
+                                               T dt) {
+  using H = decltype(row.gd->x);
+  if (slot < 0) {
+    // new contact: appended in list order (old is zero)
+    if (row.n < row.cap) {
+      slot = row.n;
+      st_nt(key, row.gkey + (size_t)slot * row.N + row.s);
+      row.remember(key);
+      ++row.n;
+    } else {
+      err |= ovf_bit;
+    }
   }
+  if (slot >= 0) row.marks |= (uint64_t)1 << slot;
+  V3<T> dn;
+  const V3<T> ft = tangential_force<T>(old, fn, vt, n, eta, k, mu, dt, dn);
+  if (slot >= 0) store_delta<H, T>(row.gd + (size_t)slot * row.N + row.s, dn);
   return ft;
 }
 
@@ -341,17 +388,25 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // ---------------------------------------------------------------------------
 // fused contact forces + integration for one step
 // ---------------------------------------------------------------------------
-// Touching pairs evaluated per round of pass 2 (their loads overlap).
-#ifndef GDEM_PAIRS_PER_ROUND
-#define GDEM_PAIRS_PER_ROUND 1
-#endif
-constexpr int PAIRS_PER_ROUND = GDEM_PAIRS_PER_ROUND;
 
 // PRow/WRow: history row types for pairs and walls.  Pair candidates are
 // screened first (indices and positions fetched eight at a time so their
 // latencies overlap, FP64 overlap test), then the touching ones are evaluated
 // in list order.  The occupancy bound of 4 waves per SIMD costs the FP64 form
 // a few spilled registers and pays for itself (measured on MI355X).
+// Occupancy bound of the step kernels (waves per SIMD), per precision: the
+// FP64 form's wider registers would spill at the mixed form's bound.
+#ifndef GDEM_FP64_WAVES
+#define GDEM_FP64_WAVES 3
+#endif
+#ifndef GDEM_MIXED_WAVES
+#define GDEM_MIXED_WAVES 3
+#endif
+template <class T>
+constexpr int step_waves() {
+  return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
+}
+
 template <int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
@@ -360,9 +415,12 @@ struct StepLds {
 };
 
 template <class Row>
-__device__ __forceinline__ void bind_lds(Row&, int*) {}
+__device__ __forceinline__ void bind_lds(Row&, int*, int) {}
 template <class H, int CAP>
-__device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col) { r.k = col; }
+__device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col0, int tx) {
+  r.k0 = col0;
+  r.k = col0 + tx;
+}
 
 template <class Row>
 struct RowCap {
@@ -406,186 +464,339 @@ struct StepArgs {
   double threshSq;
 };
 
+// ---- wave-level helpers (64 lanes) ------------------------------------------
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Lane owning work item g: the first lane whose inclusive count exceeds g
+// (incl is nondecreasing over the lanes; the result is 0..63 for any g).
+__device__ __forceinline__ int owner_lane(int incl, int g) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int v = __shfl(incl, lo + step - 1);
+    if (v <= g) lo += step;
+  }
+  return lo;
+}
+
+// Position of the k-th (from 0) set bit of m.
+__device__ __forceinline__ int nth_bit(uint64_t m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    const uint64_t low = m & ((1ull << w) - 1);
+    const int c = __popcll(low);
+    if (k >= c) {
+      k -= c;
+      m >>= w;
+      pos += w;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
+  const unsigned lo = __shfl((unsigned)x, src), hi = __shfl((unsigned)(x >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// lanes [lo, hi) as a mask
+__device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
+  if (hi <= lo) return 0;
+  const uint64_t upto = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
+  const uint64_t below = lo <= 0 ? 0ull : ((1ull << lo) - 1);
+  return upto & ~below;
+}
+
 // Contact forces, history update, integration, out-of-bounds and refresh test
-// of the particle in slot s (one thread).  PRow/WRow are the history row types
-// for pairs and walls.  Pair candidates are screened first (indices and
-// positions fetched eight at a time so their latencies overlap, FP64 overlap
-// test), then the touching ones are evaluated in list order.
+// of the particle in slot s, one thread per particle; every lane of the wave
+// calls it (valid: the lane holds a particle).  PRow/WRow are the history row
+// types for pairs and walls.
+//
+// Pair candidates are screened per particle (indices and positions fetched
+// eight at a time, FP64 overlap test).  The wave's touching pairs are then
+// spread over all 64 lanes, 64 at a time: each lane evaluates one pair of
+// some particle (gathers, history, contact arithmetic), and every particle
+// adds its pairs' forces in list order, so the sums are the ones a thread
+// walking its own list forms.  Wall candidates are distance-tested the same
+// way, then each particle handles its wall contacts in list order (vertex and
+// edge deduplication needs that order).
 template <class T, class H, bool REC, class PRow, class WRow, class Lds>
-__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, Lds& lds, const Rec<T>& rec,
+__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, Lds& lds, const Rec<T>& rec,
                                               uint32_t& err, bool& refresh) {
   using H4 = typename Vec4<H>::type;
   const int tx = threadIdx.x;
+  const int lane = __lane_id();
+  const int wtx = tx - lane;  // thread index of lane 0 of this wave
   const int N = A.N;
   const double4* __restrict__ pin = A.pin;
   const double4* __restrict__ vin = A.vin;
   const double4* __restrict__ win = A.win;
-  const double4 p = pin[s];
-  const double4 v = vin[s];
-  const double4 w = win[s];
-  if (p.w < 0.) {
+  double4 p = make_double4(0., 0., 0., -1.), v = make_double4(0., 0., 0., 0.), w = v;
+  if (valid) {
+    p = pin[s];
+    v = vin[s];
+    w = win[s];
+  }
+  const bool act = valid && p.w >= 0.;
+  if (valid && !act) {
     st_nt(p, A.pout + s);
     st_nt(v, A.vout + s);
     st_nt(w, A.wout + s);
-  } else {
-    const H4 mt = ld_nt(A.mat + s);
-    const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
-    const double invmoi = (double)mt.w;
-    const double ri = p.w, invm = v.w;
-    const T riT = (T)ri;
-    const T mu = (T)A.mu, dtT = (T)A.dt;
-    const V3<T> vi{(T)v.x, (T)v.y, (T)v.z};
-    const V3<T> wi{(T)w.x, (T)w.y, (T)w.z};
-    V3<T> f{T(0), T(0), T(0)}, mom{T(0), T(0), T(0)};
-    int nrec = 0;
-    const int self_id = (int)w.w;
-    const int nnb = ld_nt(A.nnb + s);
-    const int nwl = ld_nt(A.nwl + s);
-    const double4 rf = ld_nt(A.ref + s);  // position at the last list build
+  }
+  H4 mt{};
+  int nnb = 0, nwl = 0, nh0 = 0, nw0 = 0;
+  double4 rf = p;
+  if (act) {
+    mt = ld_nt(A.mat + s);
+    nnb = ld_nt(A.nnb + s);
+    nwl = ld_nt(A.nwl + s);
+    rf = ld_nt(A.ref + s);  // position at the last list build
+    nh0 = ld_nt(A.nh + s);
+    nw0 = ld_nt(A.nw + s);
+  }
+  const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
+  const double invmoi = (double)mt.w;
+  const double ri = p.w, invm = v.w;
+  const T riT = (T)ri;
+  const T mu = (T)A.mu, dtT = (T)A.dt;
+  const V3<T> vi{(T)v.x, (T)v.y, (T)v.z};
+  const V3<T> wi{(T)w.x, (T)w.y, (T)w.z};
+  V3<T> f{T(0), T(0), T(0)}, mom{T(0), T(0), T(0)};
+  int nrec = 0;
+  const int self_id = (int)w.w;
 
-    // ---- particle pairs -------------------------------------------------
-    {
-      PRow row;
-      bind_lds(row, &lds.pkey[0][tx]);
-      row.gkey = A.hkey;
-      row.gd = A.hd;
-      row.s = s;
-      row.N = N;
-      row.cap = A.hist_cap;
-      row.load(ld_nt(A.nh + s));
-      // pass 1: which candidates overlap (FP64 test of the reference, list
-      // order); candidates are fetched eight at a time so latencies overlap
-      uint64_t hits = 0;
-      for (int q0 = 0; q0 < nnb; q0 += 8) {
-        int jv[8];
-        double4 cv[8];
+  // ---- particle pairs -------------------------------------------------------
+  {
+    PRow row;
+    bind_lds(row, &lds.pkey[0][0], tx);
+    row.gkey = A.hkey;
+    row.gd = A.hd;
+    row.s = s;
+    row.N = N;
+    row.cap = A.hist_cap;
+    row.load(nh0, act);
+    // the keys are read from other lanes' columns below
+    __builtin_amdgcn_wave_barrier();
+    // screening: which candidates overlap (FP64 test of the reference)
+    uint64_t hits = 0;
+    for (int q0 = 0; q0 < nnb; q0 += 8) {
+      int jv[8];
+      double4 cv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) jv[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s].x : -1;
+      for (int u = 0; u < 8; ++u) jv[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s].x : -1;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pin[jv[u]] : make_double4(0., 0., 0., -1.);
+      for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pin[jv[u]] : make_double4(0., 0., 0., -1.);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const double4 pj = cv[u];
-          if (pj.w < 0.) continue;
-          const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-          const double distsq = dx * dx + dy * dy + dz * dz;
-          const double R = ri + pj.w;
-          if (distsq < R * R) hits |= (uint64_t)1 << (q0 + u);
-        }
+      for (int u = 0; u < 8; ++u) {
+        const double4 pj = cv[u];
+        if (pj.w < 0.) continue;
+        const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
+        const double distsq = dx * dx + dy * dy + dz * dz;
+        const double R = ri + pj.w;
+        if (distsq < R * R) hits |= (uint64_t)1 << (q0 + u);
       }
-      // pass 2: the touching pairs in list order, two at a time.  Their
-      // history slots are looked up first (new slots are still appended in
-      // list order), so position, velocity, spin and history of both are
-      // fetched in one round.
-      while (hits) {
-        const int qa = __ffsll((unsigned long long)hits) - 1;
-        hits &= hits - 1;
-        const int qb = (PAIRS_PER_ROUND > 1 && hits) ? __ffsll((unsigned long long)hits) - 1 : -1;
-        if (qb >= 0) hits &= hits - 1;
-        int2 e0 = A.nbr[(size_t)qa * N + s];
-        int2 e1 = qb >= 0 ? A.nbr[(size_t)qb * N + s] : make_int2(-1, -1);
-        int s0 = row.find(e0.y);
-        int s1 = e1.x >= 0 ? row.find(e1.y) : -1;
-        double4 pj0 = pin[e0.x], vj0 = vin[e0.x], wj0 = win[e0.x];
-        H4 d0 = s0 >= 0 ? ld_nt(row.gd + (size_t)s0 * N + s) : H4{};
-        double4 pj1 = make_double4(0., 0., 0., 0.), vj1 = pj1, wj1 = pj1;
-        H4 d1{};
-        if (e1.x >= 0) {
-          pj1 = pin[e1.x];
-          vj1 = vin[e1.x];
-          wj1 = win[e1.x];
-          if (s1 >= 0) d1 = ld_nt(row.gd + (size_t)s1 * N + s);
-        }
-        for (int u = 0; u < 2; ++u) {
-          if (u == 1) {
-            if (e1.x < 0) break;
-            e0 = e1;
-            s0 = s1;
-            pj0 = pj1;
-            vj0 = vj1;
-            wj0 = wj1;
-            d0 = d1;
-          }
-          const double4 pj = pj0, vj = vj0, wj = wj0;
-          const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
-          const double distsq = dx * dx + dy * dy + dz * dz;
-          const double R = ri + pj.w;
-          const double dist = sqrt(distsq);
-          const double delMagD = R - dist;
-          V3<T> n;
-          if constexpr (sizeof(T) == 8) {
-            n = V3<T>{dx / dist, dy / dist, dz / dist};
+    }
+    // the touching pairs, spread over the lanes
+    const int cnt = __popcll(hits);
+    const int incl = wave_incl_sum(cnt);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+    const int first = incl - cnt;
+    int added = 0;  // new history slots this particle's pairs have asked for
+    uint64_t marks = 0;
+    for (int base = 0; base < total; base += 64) {
+      const int g = base + lane;
+      const bool has = g < total;
+      const int o = owner_lane(incl, g);
+      const int o_first = __shfl(first, o);
+      const uint64_t o_hits = shfl64(hits, o);
+      const int so = __shfl(s, o);
+      const int o_n = __shfl(nh0, o);
+      const int o_added = __shfl(added, o);
+      const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
+      const double ori = __shfl(ri, o), oinvm = __shfl(invm, o);
+      const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
+      const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
+      const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
+      int2 e = make_int2(0, -1);
+      int slot = -1;
+      double4 pj = make_double4(0., 0., 0., 0.), vj = pj, wj = pj;
+      H4 d{};
+      if (has) {
+        const int q = nth_bit(o_hits, g - o_first);
+        e = A.nbr[(size_t)q * N + so];
+        slot = row.find_in(wtx + o, so, o_n, e.y);
+        pj = pin[e.x];
+        vj = vin[e.x];
+        wj = win[e.x];
+        if (slot >= 0) d = ld_nt(row.gd + (size_t)slot * N + so);
+      }
+      // new contacts take their particle's next free slots in list order
+      const uint64_t fresh = __ballot(has && slot < 0);
+      const uint64_t before_me = lane_range(max(o_first - base, 0), lane);
+      V3<T> fn{T(0), T(0), T(0)}, ft{T(0), T(0), T(0)}, mm{T(0), T(0), T(0)};
+      double delMagD = 0.;
+      int wslot = -1;
+      if (has) {
+        wslot = slot;
+        if (slot < 0) {
+          const int ns = o_n + o_added + __popcll(fresh & before_me);
+          if (ns < row.cap) {
+            wslot = ns;
+            st_nt(e.y, row.gkey + (size_t)ns * N + so);
           } else {
-            const T inv = T(1) / (T)dist;
-            n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
+            err |= GDEM_ERR_HIST_OVERFLOW_BIT;
           }
-          const T delMag = (T)delMagD;
-          const V3<T> del = scal(-delMag, n);
-          const V3<T> vr{vi.x - (T)vj.x, vi.y - (T)vj.y, vi.z - (T)vj.z};
-          const T vrn = dot(vr, n);
-          const V3<T> vn = scal(vrn, n);
-          const T meff = T(1) / ((T)invm + (T)vj.w);
-          const T eta = etac * sqrt(meff);
-          V3<T> fn;
-          fn.x = -k * del.x - eta * vn.x;
-          fn.y = -k * del.y - eta * vn.y;
-          fn.z = -k * del.z - eta * vn.z;
-          const T rj = (T)pj.w;
-          V3<T> vrot{riT * wi.x + rj * (T)wj.x, riT * wi.y + rj * (T)wj.y, riT * wi.z + rj * (T)wj.z};
-          vrot = cross(vrot, n);
-          const V3<T> vt = vadd(vsub(vr, vn), vrot);
-          f.x += fn.x;
-          f.y += fn.y;
-          f.z += fn.z;
-          const int key = e0.y;  // original id of j
-          const V3<T> ft = tangential_at<T>(row, err, GDEM_ERR_HIST_OVERFLOW_BIT, key, s0,
-                                               V3<T>{(T)d0.x, (T)d0.y, (T)d0.z}, fn, vt, n, eta, k, mu, dtT);
-          f.x += ft.x;
-          f.y += ft.y;
-          f.z += ft.z;
-          const V3<T> mm = scal(riT, cross(n, ft));
-          mom.x += mm.x;
-          mom.y += mm.y;
-          mom.z += mm.z;
+        }
+        const double dx = opx - pj.x, dy = opy - pj.y, dz = opz - pj.z;
+        const double distsq = dx * dx + dy * dy + dz * dz;
+        const double R = ori + pj.w;
+        const double dist = sqrt(distsq);
+        delMagD = R - dist;
+        V3<T> n;
+        if constexpr (sizeof(T) == 8) {
+          n = V3<T>{dx / dist, dy / dist, dz / dist};
+        } else {
+          const T inv = T(1) / (T)dist;
+          n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
+        }
+        const T delMag = (T)delMagD;
+        const V3<T> del = scal(-delMag, n);
+        const V3<T> vr{ovi.x - (T)vj.x, ovi.y - (T)vj.y, ovi.z - (T)vj.z};
+        const T vrn = dot(vr, n);
+        const V3<T> vn = scal(vrn, n);
+        const T meff = T(1) / ((T)oinvm + (T)vj.w);
+        const T eta = oetac * sqrt(meff);
+        fn.x = -ok * del.x - eta * vn.x;
+        fn.y = -ok * del.y - eta * vn.y;
+        fn.z = -ok * del.z - eta * vn.z;
+        const T rj = (T)pj.w;
+        V3<T> vrot{oriT * owi.x + rj * (T)wj.x, oriT * owi.y + rj * (T)wj.y, oriT * owi.z + rj * (T)wj.z};
+        vrot = cross(vrot, n);
+        const V3<T> vt = vadd(vsub(vr, vn), vrot);
+        const V3<T> old = slot >= 0 ? V3<T>{(T)d.x, (T)d.y, (T)d.z} : V3<T>{T(0), T(0), T(0)};
+        V3<T> dn;
+        ft = tangential_force<T>(old, fn, vt, n, eta, ok, mu, dtT, dn);
+        if (wslot >= 0) store_delta<H, T>(row.gd + (size_t)wslot * N + so, dn);
+        mm = scal(oriT, cross(n, ft));
+      }
+      // every particle adds its pairs of this round in list order
+      const int my_lo = max(first - base, 0), my_hi = min(first + cnt - base, 64);
+      const int my_n = max(my_hi - my_lo, 0);
+      const int rounds = wave_max(my_n);
+      for (int t = 0; t < rounds; ++t) {
+        const bool mine = t < my_n;
+        const int src = mine ? my_lo + t : lane;
+        const T a0 = __shfl(fn.x, src), a1 = __shfl(fn.y, src), a2 = __shfl(fn.z, src);
+        const T b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        const T c0 = __shfl(mm.x, src), c1 = __shfl(mm.y, src), c2 = __shfl(mm.z, src);
+        const int ws = __shfl(wslot, src);
+        int rkey = 0;
+        double rdel = 0.;
+        if constexpr (REC) {
+          rkey = __shfl(e.y, src);
+          rdel = __shfl(delMagD, src);
+        }
+        if (mine) {
+          f.x += a0;
+          f.y += a1;
+          f.z += a2;
+          f.x += b0;
+          f.y += b1;
+          f.z += b2;
+          mom.x += c0;
+          mom.y += c1;
+          mom.z += c2;
+          if (ws >= 0) marks |= (uint64_t)1 << ws;
           if constexpr (REC) {
             if (nrec < rec.cap) {
-              const size_t o = (size_t)self_id * rec.cap + nrec;
-              rec.key[o] = key;
-              rec.kind[o] = 0;
-              rec.fn[o * 3] = fn.x; rec.fn[o * 3 + 1] = fn.y; rec.fn[o * 3 + 2] = fn.z;
-              rec.ft[o * 3] = ft.x; rec.ft[o * 3 + 1] = ft.y; rec.ft[o * 3 + 2] = ft.z;
-              rec.delta[o] = delMagD;
+              const size_t r = (size_t)self_id * rec.cap + nrec;
+              rec.key[r] = rkey;
+              rec.kind[r] = 0;
+              rec.fn[r * 3] = a0; rec.fn[r * 3 + 1] = a1; rec.fn[r * 3 + 2] = a2;
+              rec.ft[r * 3] = b0; rec.ft[r * 3 + 1] = b1; rec.ft[r * 3 + 2] = b2;
+              rec.delta[r] = rdel;
               ++nrec;
             }
           }
         }
       }
-      st_nt(truncate_history(row.n, row.marks), A.nh + s);
+      added += __popcll(fresh & lane_range(my_lo, my_hi));
     }
+    row.n = min(nh0 + added, row.cap);
+    row.marks = marks;
+    if (act) st_nt(truncate_history(row.n, row.marks), A.nh + s);
+  }
 
-    // ---- walls ------------------------------------------------------------
-    {
-      WRow row;
-      bind_lds(row, &lds.wkey[0][tx]);
-      row.gkey = A.wkey;
-      row.gd = A.wd;
-      row.s = s;
-      row.N = N;
-      row.cap = A.whist_cap;
-      row.load(ld_nt(A.nw + s));
-      int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
-      int nve = 0;
-      const double rsq = ri * ri;
-      const T eta = etac * sqrtm;
-      for (int q = 0; q < nwl; ++q) {
-        const int t = ld_nt(A.wl + (size_t)q * N + s);
-        const TriHit h = tri_distance(A.tri + t, p.x, p.y, p.z, rsq);
-        if (!(h.dist < ri)) continue;
+  // ---- walls ----------------------------------------------------------------
+  {
+    WRow row;
+    bind_lds(row, &lds.wkey[0][0], tx);
+    row.gkey = A.wkey;
+    row.gd = A.wd;
+    row.s = s;
+    row.N = N;
+    row.cap = A.whist_cap;
+    row.load(nw0, act);
+    int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
+    int nve = 0;
+    const double rsq = ri * ri;
+    const T eta = etac * sqrtm;
+    // distance tests of the wave's wall candidates, spread over the lanes
+    const int incl = wave_incl_sum(nwl);
+    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+    const int first = incl - nwl;
+    for (int base = 0; base < total; base += 64) {
+      const int g = base + lane;
+      const bool has = g < total;
+      const int o = owner_lane(incl, g);
+      const int o_first = __shfl(first, o);
+      const int so = __shfl(s, o);
+      const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
+      const double orsq = __shfl(rsq, o);
+      int t = 0;
+      TriHit th;
+      th.n = V3<double>{0., 0., 0.};
+      th.dist = 1.e10;
+      th.hit = -1;
+      if (has) {
+        t = ld_nt(A.wl + (size_t)(g - o_first) * N + so);
+        th = tri_distance(A.tri + t, opx, opy, opz, orsq);
+      }
+      // every particle handles its candidates of this round in list order
+      const int my_lo = max(first - base, 0), my_hi = min(first + nwl - base, 64);
+      const int my_n = max(my_hi - my_lo, 0);
+      const int rounds = wave_max(my_n);
+      for (int u = 0; u < rounds; ++u) {
+        const bool mine = u < my_n;
+        const int src = mine ? my_lo + u : lane;
+        TriHit h;
+        h.n = V3<double>{__shfl(th.n.x, src), __shfl(th.n.y, src), __shfl(th.n.z, src)};
+        h.dist = __shfl(th.dist, src);
+        h.hit = __shfl(th.hit, src);
+        const int tt = __shfl(t, src);
+        if (!mine || !(h.dist < ri)) continue;
         if (h.hit != -1) {
           bool dup = false;
           const int seen = nve < 8 ? nve : 8;
-          for (int u = 0; u < seen; ++u)
-            if (ve[u * 256] == h.hit) {
+          for (int z = 0; z < seen; ++z)
+            if (ve[z * 256] == h.hit) {
               dup = true;
               break;
             }
@@ -610,7 +821,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, Lds& 
         f.x += fn.x;
         f.y += fn.y;
         f.z += fn.z;
-        const V3<T> ft = tangential<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, t, fn, vt, n, eta, k, mu, dtT);
+        const V3<T> ft = tangential<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, tt, fn, vt, n, eta, k, mu, dtT);
         f.x += ft.x;
         f.y += ft.y;
         f.z += ft.z;
@@ -620,54 +831,57 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, Lds& 
         mom.z += mm.z;
         if constexpr (REC) {
           if (nrec < rec.cap) {
-            const size_t o = (size_t)self_id * rec.cap + nrec;
-            rec.key[o] = t;
-            rec.kind[o] = 1;
-            rec.fn[o * 3] = fn.x; rec.fn[o * 3 + 1] = fn.y; rec.fn[o * 3 + 2] = fn.z;
-            rec.ft[o * 3] = ft.x; rec.ft[o * 3 + 1] = ft.y; rec.ft[o * 3 + 2] = ft.z;
-            rec.delta[o] = delMagD;
+            const size_t r = (size_t)self_id * rec.cap + nrec;
+            rec.key[r] = tt;
+            rec.kind[r] = 1;
+            rec.fn[r * 3] = fn.x; rec.fn[r * 3 + 1] = fn.y; rec.fn[r * 3 + 2] = fn.z;
+            rec.ft[r * 3] = ft.x; rec.ft[r * 3 + 1] = ft.y; rec.ft[r * 3 + 2] = ft.z;
+            rec.delta[r] = delMagD;
             ++nrec;
           }
         }
       }
-      st_nt(truncate_history(row.n, row.marks), A.nw + s);
     }
-    if constexpr (REC) rec.count[self_id] = nrec;
-
-    // ---- integration (k_integrate), out-of-bounds, refresh check ----------
-    const double dt = A.dt;
-    const double ax = (double)f.x * invm + A.g[0];
-    const double ay = (double)f.y * invm + A.g[1];
-    const double az = (double)f.z * invm + A.g[2];
-    double4 vo, po, wo;
-    vo.x = v.x + ax * dt;
-    vo.y = v.y + ay * dt;
-    vo.z = v.z + az * dt;
-    vo.w = v.w;
-    po.x = p.x + vo.x * dt;
-    po.y = p.y + vo.y * dt;
-    po.z = p.z + vo.z * dt;
-    const double bx = (double)mom.x * invmoi;
-    const double by = (double)mom.y * invmoi;
-    const double bz = (double)mom.z * invmoi;
-    wo.x = w.x + bx * dt;
-    wo.y = w.y + by * dt;
-    wo.z = w.z + bz * dt;
-    wo.w = w.w;
-    const bool out = po.x < A.bmin[0] || po.x > A.bmax[0] || po.y < A.bmin[1] ||
-                     po.y > A.bmax[1] || po.z < A.bmin[2] || po.z > A.bmax[2];
-    po.w = out ? -ri : ri;
-    if (!out) {
-      const double ex = po.x - rf.x, ey = po.y - rf.y, ez = po.z - rf.z;
-      refresh = (ex * ex + ey * ey + ez * ez) > A.threshSq;
-    }
-    if (!(isfinite(po.x) && isfinite(po.y) && isfinite(po.z) && isfinite(vo.x) &&
-          isfinite(vo.y) && isfinite(vo.z)))
-      err |= GDEM_ERR_NAN_BIT;
-    st_nt(po, A.pout + s);
-    st_nt(vo, A.vout + s);
-    st_nt(wo, A.wout + s);
+    if (act) st_nt(truncate_history(row.n, row.marks), A.nw + s);
   }
+  if constexpr (REC) {
+    if (act) rec.count[self_id] = nrec;
+  }
+  if (!act) return;
+
+  // ---- integration (k_integrate), out-of-bounds, refresh check --------------
+  const double dt = A.dt;
+  const double ax = (double)f.x * invm + A.g[0];
+  const double ay = (double)f.y * invm + A.g[1];
+  const double az = (double)f.z * invm + A.g[2];
+  double4 vo, po, wo;
+  vo.x = v.x + ax * dt;
+  vo.y = v.y + ay * dt;
+  vo.z = v.z + az * dt;
+  vo.w = v.w;
+  po.x = p.x + vo.x * dt;
+  po.y = p.y + vo.y * dt;
+  po.z = p.z + vo.z * dt;
+  const double bx = (double)mom.x * invmoi;
+  const double by = (double)mom.y * invmoi;
+  const double bz = (double)mom.z * invmoi;
+  wo.x = w.x + bx * dt;
+  wo.y = w.y + by * dt;
+  wo.z = w.z + bz * dt;
+  wo.w = w.w;
+  const bool out = po.x < A.bmin[0] || po.x > A.bmax[0] || po.y < A.bmin[1] || po.y > A.bmax[1] ||
+                   po.z < A.bmin[2] || po.z > A.bmax[2];
+  po.w = out ? -ri : ri;
+  if (!out) {
+    const double ex = po.x - rf.x, ey = po.y - rf.y, ez = po.z - rf.z;
+    refresh = (ex * ex + ey * ey + ez * ez) > A.threshSq;
+  }
+  if (!(isfinite(po.x) && isfinite(po.y) && isfinite(po.z) && isfinite(vo.x) && isfinite(vo.y) &&
+        isfinite(vo.z)))
+    err |= GDEM_ERR_NAN_BIT;
+  st_nt(po, A.pout + s);
+  st_nt(vo, A.vout + s);
+  st_nt(wo, A.wout + s);
 }
 
 // What the candidate-list builders read and write.  Cell ids are global:

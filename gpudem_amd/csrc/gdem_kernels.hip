@@ -18,10 +18,8 @@ namespace gdem {
 // ---------------------------------------------------------------------------
 // fused contact forces + integration for one step
 // ---------------------------------------------------------------------------
-// The occupancy bound of 4 waves per SIMD costs the FP64 form a few spilled
-// registers and pays for itself (measured on MI355X).
 template <class T, class H, bool REC, class PRow, class WRow>
-__global__ __launch_bounds__(256, 4) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
+__global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
   __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int cur = __builtin_amdgcn_readfirstlane(B.ctl[CTL_CUR]);
@@ -59,7 +57,7 @@ __global__ __launch_bounds__(256, 4) void k_step(SimParams P, Bufs<H> B, Rec<T> 
   A.threshSq = P.threshSq;
   uint32_t err = 0;
   bool refresh = false;
-  if (s < P.N) step_particle<T, H, REC, PRow, WRow>(A, s, lds, rec, err, refresh);
+  step_particle<T, H, REC, PRow, WRow>(A, s, s < P.N, lds, rec, err, refresh);
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
   if (err) atomicOr((unsigned int*)&B.ctl[CTL_ERR], err);
   if (last_block(&B.ctl[CTL_DONE_STEP])) {
