@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=0,
                     help="> 0: launch-chain rebuild captured in graphs of this many steps; "
                          "< 0: launch-chain rebuild, direct launches")
+    ap.add_argument("--skin-factor", type=float, default=None,
+                    help="chute Verlet skin in units of r_max (refresh = skin/2, cell = 2 r_max + skin)")
     ap.add_argument("--wall-broadphase", type=int, default=0,
                     help="0: per-cell table when it fits, 1: BVH traversal, 2: per-cell table")
     ap.add_argument("--sample-every", type=int, default=10,
@@ -219,7 +221,8 @@ def main():
     import numpy as np
 
     from gpudem_amd import GDEM_FP64, GDEM_MIXED, Context, scenes
-    spec = scenes.helical_chute(n=args.particles)
+    spec = scenes.helical_chute(n=args.particles) if args.skin_factor is None else \
+        scenes.helical_chute(n=args.particles, skin_factor=args.skin_factor)
     spec.params["wall_broadphase"] = args.wall_broadphase
     print(f"# workload helical_chute n={spec.n} ntri={len(spec.verts)}", file=sys.stderr, flush=True)
     state = None

@@ -914,27 +914,45 @@ struct ListArgs {
 // all active j != i in the 27 surrounding cells with d.d < (r_i+r_j+skin)^2,
 // ordered by original id.  Also records the reference position.
 constexpr int NB_MAX = 64;
+constexpr int NB_REG = 32;  // capacities up to this keep the list in registers
 
-__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active) {
+// Inserts (id, q) into the first n entries of kp/kq, kept ascending in id.
+// Every index is a compile-time constant, so the arrays stay in registers.
+template <int R>
+__device__ __forceinline__ void insert_sorted(int (&kp)[R], int (&kq)[R], int n, int id, int q) {
+  int at = 0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) at += (k < n && kp[k] < id) ? 1 : 0;
+#pragma unroll
+  for (int k = R - 1; k > 0; --k) {
+    const bool up = k > at && k <= n;
+    kp[k] = up ? kp[k - 1] : kp[k];
+    kq[k] = up ? kq[k - 1] : kq[k];
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    kp[k] = k == at ? id : kp[k];
+    kq[k] = k == at ? q : kq[k];
+  }
+}
+
+// R: size of the per-thread candidate arrays (>= the capacity used).
+template <int R>
+__device__ __forceinline__ void pairlist_core(const ListArgs& L, int s) {
   const int N = L.N;
   const double4* __restrict__ pos = L.pos;
   const int* __restrict__ pid = L.pid;
   const double4 p = pos[s];
-  L.ref[s] = p;
-  if (!active) {
-    L.nnb[s] = 0;
-    return;
-  }
   const int c = L.cell_of[s] - L.cell_base;
   const int sx = L.size[0], sy = L.size[1], sz = L.size[2];
   const int cx = c % sx;
   const int cy = (c / sx) % sy;
   const int cz = c / (sx * sy);
-  int cand[NB_MAX];
-  int cpid[NB_MAX];
+  int cand[R];
+  int cpid[R];
   int n = 0;
   bool ovf = false;
-  const int cap = L.nb_cap < NB_MAX ? L.nb_cap : NB_MAX;
+  const int cap = L.nb_cap < R ? L.nb_cap : R;
   for (int dz = -1; dz <= 1; ++dz) {
     const int Z = cz + dz;
     if (Z < 0 || Z >= sz) continue;
@@ -953,30 +971,50 @@ __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool acti
           const double4 pj = pos[q];
           const double ddx = p.x - pj.x, ddy = p.y - pj.y, ddz = p.z - pj.z;
           const double distsq = ddx * ddx + ddy * ddy + ddz * ddz;
-          const double R = p.w + pj.w + L.skin;
-          if (distsq < R * R) {
+          const double R2 = p.w + pj.w + L.skin;
+          if (distsq < R2 * R2) {
             if (n >= cap) {
               ovf = true;
               continue;
             }
             const int id = pid[q];
-            int at = n;
-            while (at > 0 && cpid[at - 1] > id) {
-              cpid[at] = cpid[at - 1];
-              cand[at] = cand[at - 1];
-              --at;
+            if constexpr (R <= NB_REG) {
+              insert_sorted<R>(cpid, cand, n, id, q);
+            } else {
+              int at = n;
+              while (at > 0 && cpid[at - 1] > id) {
+                cpid[at] = cpid[at - 1];
+                cand[at] = cand[at - 1];
+                --at;
+              }
+              cpid[at] = id;
+              cand[at] = q;
             }
-            cpid[at] = id;
-            cand[at] = q;
             ++n;
           }
         }
       }
     }
   }
-  for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+  if constexpr (R <= NB_REG) {
+#pragma unroll
+    for (int q = 0; q < R; ++q)
+      if (q < n) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+  } else {
+    for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+  }
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
+}
+
+__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active) {
+  L.ref[s] = L.pos[s];
+  if (!active) {
+    L.nnb[s] = 0;
+    return;
+  }
+  if (L.nb_cap <= NB_REG) pairlist_core<NB_REG>(L, s);
+  else pairlist_core<NB_MAX>(L, s);
 }
 
 // Wall candidates by BVH traversal (k_update_neighborlist_wall,
@@ -1009,7 +1047,8 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
     return (ex * ex + ey * ey + ez * ez) <= Rsq;
   };
   if (L.wall_table) {
-    // static per-cell table: ascending triangle ids already
+    // static per-cell table: ascending triangle ids already, so candidates
+    // go straight to their slots
     const int c = L.cell_of[s];
     const int b1 = L.wc_off[c + 1];
     for (int q = L.wc_off[c]; q < b1; ++q) {
@@ -1022,9 +1061,13 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
           ovf = true;
           continue;
         }
-        list[n++] = t;
+        L.wl[(size_t)n * N + s] = t;
+        ++n;
       }
     }
+    L.nwl[s] = n;
+    if (ovf) atomicOr(L.err, GDEM_ERR_WALL_OVERFLOW_BIT);
+    return;
   } else {
     int stack[64];
     int sp = 0;
