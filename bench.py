@@ -206,6 +206,7 @@ def main():
     ap.add_argument("--served-requests", type=int, default=60)
     ap.add_argument("--served-budget-gb", type=float, default=8.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 error report")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (skin 0.1, cell 1.0, graph 100)")
     ap.add_argument("--batch-graph-steps", type=int, default=None,
@@ -280,6 +281,12 @@ def main():
         out["fp64_roofline"] = {"achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": d["achieved"] / HBM_PEAK_GBS, "bytes_per_particle_step": d["B"],
                                 "avg_launch_us": d["force_avg_s"] * 1e6, "kernel": "k_step<double,double>"}
+    if not args.no_accuracy:
+        from gpudem_amd import accuracy
+        acc = accuracy.report(ctx)
+        log(f"# mixed vs fp64: force {acc['force_rel_err']:.2e}, restitution {acc['restitution_rel_err']:.2e}, "
+            f"packing {acc['packing_rel_err']:.2e}")
+        out["mixed_precision_error"] = acc
     if not args.no_batch:
         from gpudem_amd import GDEM_FP64 as F64, autotune
         mix = scenes.batch_mix(nscenes=args.batch_scenes, steps=args.batch_steps)

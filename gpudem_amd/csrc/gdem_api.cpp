@@ -320,6 +320,14 @@ int record_impl(gdem_scene* sc, gdem_contact_view* vw) {
   HIPCHK(c, tmp.alloc(&R.fn, N * cap * 3));
   HIPCHK(c, tmp.alloc(&R.ft, N * cap * 3));
   HIPCHK(c, tmp.alloc(&R.delta, N * cap));
+  if (vw->accel) {
+    HIPCHK(c, tmp.alloc(&R.acc, N * 3));
+    HIPCHK(c, hipMemsetAsync(R.acc, 0, N * 24, c->stream));
+  }
+  if (vw->ang_accel) {
+    HIPCHK(c, tmp.alloc(&R.anga, N * 3));
+    HIPCHK(c, hipMemsetAsync(R.anga, 0, N * 24, c->stream));
+  }
   HIPCHK(c, hipMemsetAsync(R.count, 0, N * 4, c->stream));
   launch_step_record<H, H>(sc->P, bufs<H>(sc), R, c->stream);
   HIPCHK(c, launch_rebuild<H>(sc->P, bufs<H>(sc), sc->coop, c->stream));
@@ -336,7 +344,40 @@ int record_impl(gdem_scene* sc, gdem_contact_view* vw) {
   rc |= cp(vw->fn, R.fn, N * cap * 24);
   rc |= cp(vw->ft, R.ft, N * cap * 24);
   rc |= cp(vw->delta, R.delta, N * cap * 8);
+  if (vw->accel) rc |= cp(vw->accel, R.acc, N * 24);
+  if (vw->ang_accel) rc |= cp(vw->ang_accel, R.anga, N * 24);
   return rc ? GDEM_E_HIP : GDEM_OK;
+}
+
+template <class H>
+int set_history_impl(gdem_scene* sc, const gdem_history_view* vw) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  const SimParams& P = sc->P;
+  for (size_t i = 0; i < N; ++i)
+    if (vw->nhist[i] < 0 || vw->nhist[i] > P.hist_cap || vw->nwhist[i] < 0 || vw->nwhist[i] > P.whist_cap)
+      return fail(c, GDEM_E_SLOT_OVERFLOW, "history row longer than the scene's capacity");
+  DevArena tmp;
+  int *nh, *hk, *nw, *wk;
+  double *hd, *wd;
+  HIPCHK(c, tmp.alloc(&nh, N));
+  HIPCHK(c, tmp.alloc(&hk, N * P.hist_cap));
+  HIPCHK(c, tmp.alloc(&hd, N * P.hist_cap * 3));
+  HIPCHK(c, tmp.alloc(&nw, N));
+  HIPCHK(c, tmp.alloc(&wk, N * P.whist_cap));
+  HIPCHK(c, tmp.alloc(&wd, N * P.whist_cap * 3));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(nh, vw->nhist, N * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(hk, vw->hist_key, N * P.hist_cap * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(hd, vw->hist_delta, N * P.hist_cap * 24, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(nw, vw->nwhist, N * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(wk, vw->whist_key, N * P.whist_cap * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(wd, vw->whist_delta, N * P.whist_cap * 24, hipMemcpyHostToDevice));
+  HistIn in{nh, hk, hd, nw, wk, wd};
+  launch_set_history<H>(P, bufs<H>(sc), in, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GDEM_OK;
 }
 
 template <class H>
@@ -606,6 +647,13 @@ int gdem_scene_debug(gdem_scene* sc, gdem_debug_view* vw) {
 int gdem_scene_step_record(gdem_scene* sc, gdem_contact_view* vw) {
   if (!sc || !vw || vw->rec_cap <= 0) return GDEM_E_ARG;
   return sc->precision == GDEM_FP64 ? record_impl<double>(sc, vw) : record_impl<float>(sc, vw);
+}
+
+int gdem_scene_set_history(gdem_scene* sc, const gdem_history_view* vw) {
+  if (!sc || !vw || !vw->nhist || !vw->hist_key || !vw->hist_delta || !vw->nwhist || !vw->whist_key ||
+      !vw->whist_delta)
+    return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? set_history_impl<double>(sc, vw) : set_history_impl<float>(sc, vw);
 }
 
 int gdem_scene_set_timing(gdem_scene* sc, int on) {

@@ -372,6 +372,8 @@ struct Rec {
   double* fn;
   double* ft;
   double* delta;
+  double* acc;   // [n*3] or null: linear acceleration of the step
+  double* anga;  // [n*3] or null: angular acceleration of the step
 };
 
 // The hardware hands workgroup b to XCD b % 8.  Giving each XCD a contiguous
@@ -865,6 +867,18 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const double bx = (double)mom.x * invmoi;
   const double by = (double)mom.y * invmoi;
   const double bz = (double)mom.z * invmoi;
+  if constexpr (REC) {
+    if (rec.acc) {
+      rec.acc[(size_t)self_id * 3] = ax;
+      rec.acc[(size_t)self_id * 3 + 1] = ay;
+      rec.acc[(size_t)self_id * 3 + 2] = az;
+    }
+    if (rec.anga) {
+      rec.anga[(size_t)self_id * 3] = bx;
+      rec.anga[(size_t)self_id * 3 + 1] = by;
+      rec.anga[(size_t)self_id * 3 + 2] = bz;
+    }
+  }
   wo.x = w.x + bx * dt;
   wo.y = w.y + by * dt;
   wo.z = w.z + bz * dt;

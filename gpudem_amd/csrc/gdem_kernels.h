@@ -21,6 +21,18 @@ struct RecDev {
   double* fn;
   double* ft;
   double* delta;
+  double* acc;   // [n*3] or null: linear acceleration of the step
+  double* anga;  // [n*3] or null: angular acceleration of the step
+};
+
+// device copies of a gdem_history_view (original particle order)
+struct HistIn {
+  const int* nhist;
+  const int* hist_key;
+  const double* hist_delta;
+  const int* nwhist;
+  const int* whist_key;
+  const double* whist_delta;
 };
 
 struct DebugDev {
@@ -58,6 +70,8 @@ template <class H>
 void launch_debug(const SimParams& P, const Bufs<H>& B, const DebugDev& D, hipStream_t st);
 template <class H>
 void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc, hipStream_t st);
+template <class H>
+void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st);
 
 void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1,
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st);

@@ -459,6 +459,51 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
   }
 }
 
+// Contact histories from original particle order into the current internal
+// order (snapshot restore).  Rows longer than the capacity raise the
+// overflow bits and are cut at the capacity.
+template <class H>
+__global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, HistIn in) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.N) return;
+  const int N = P.N;
+  const int hc = B.ctl[CTL_HCUR];
+  const int id = B.pid[hc][s];
+  using H4 = typename Vec4<H>::type;
+  int nh = in.nhist[id];
+  if (nh > P.hist_cap) {
+    atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_HIST_OVERFLOW_BIT);
+    nh = P.hist_cap;
+  }
+  B.nh[hc][s] = nh;
+  for (int q = 0; q < nh; ++q) {
+    const size_t src = (size_t)id * P.hist_cap + q;
+    B.hkey[hc][(size_t)q * N + s] = in.hist_key[src];
+    H4 d;
+    d.x = (H)in.hist_delta[src * 3];
+    d.y = (H)in.hist_delta[src * 3 + 1];
+    d.z = (H)in.hist_delta[src * 3 + 2];
+    d.w = 0;
+    B.hd[hc][(size_t)q * N + s] = d;
+  }
+  int nw = in.nwhist[id];
+  if (nw > P.whist_cap) {
+    atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_WHIST_OVERFLOW_BIT);
+    nw = P.whist_cap;
+  }
+  B.nw[hc][s] = nw;
+  for (int q = 0; q < nw; ++q) {
+    const size_t src = (size_t)id * P.whist_cap + q;
+    B.wkey[hc][(size_t)q * N + s] = in.whist_key[src];
+    H4 d;
+    d.x = (H)in.whist_delta[src * 3];
+    d.y = (H)in.whist_delta[src * 3 + 1];
+    d.z = (H)in.whist_delta[src * 3 + 2];
+    d.w = 0;
+    B.wd[hc][(size_t)q * N + s] = d;
+  }
+}
+
 // per-particle list/contact counts for statistics
 template <class H>
 __global__ __launch_bounds__(256) void k_counts(SimParams P, Bufs<H> B, unsigned long long* acc) {
@@ -537,7 +582,7 @@ void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
 
 template <class T, class H>
 void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st) {
-  Rec<T> rec{r.cap, r.count, r.key, r.kind, r.fn, r.ft, r.delta};
+  Rec<T> rec{r.cap, r.count, r.key, r.kind, r.fn, r.ft, r.delta, r.acc, r.anga};
   k_step<T, H, true, HistScan<H>, HistScan<H>><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
 }
 
@@ -607,6 +652,11 @@ void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc
   k_counts<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, acc);
 }
 
+template <class H>
+void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st) {
+  k_set_history<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, in);
+}
+
 void launch_tri_probe(const TriGeo* tri, int np, const double* pts, const double* rsq, double* out,
                       int* hit, hipStream_t st) {
   k_tri_probe<<<blocks_for(np, 256), 256, 0, st>>>(tri, np, pts, rsq, out, hit);
@@ -631,5 +681,7 @@ template void launch_debug<double>(const SimParams&, const Bufs<double>&, const 
 template void launch_debug<float>(const SimParams&, const Bufs<float>&, const DebugDev&, hipStream_t);
 template void launch_counts<double>(const SimParams&, const Bufs<double>&, unsigned long long*, hipStream_t);
 template void launch_counts<float>(const SimParams&, const Bufs<float>&, unsigned long long*, hipStream_t);
+template void launch_set_history<double>(const SimParams&, const Bufs<double>&, const HistIn&, hipStream_t);
+template void launch_set_history<float>(const SimParams&, const Bufs<float>&, const HistIn&, hipStream_t);
 
 }  // namespace gdem

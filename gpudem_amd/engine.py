@@ -29,7 +29,9 @@ SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error"
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
            "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
            "gdem_batch_step", "gdem_batch_poll", "gdem_batch_download", "gdem_batch_scene_status",
-           "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing"]
+           "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing", "gdem_scene_set_history",
+           "gdem_settings_load", "gdem_settings_wall", "gdem_settings_particles", "gdem_output_init",
+           "gdem_write_frame_bin", "gdem_write_pid_header", "gdem_write_pid_row", "gdem_write_initial_pos"]
 
 
 class GdemError(RuntimeError):
@@ -93,7 +95,23 @@ class DebugView(ctypes.Structure):
 
 class ContactView(ctypes.Structure):
     _fields_ = [("rec_cap", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in
-                                              ("count", "key", "kind", "fn", "ft", "delta")]
+                                              ("count", "key", "kind", "fn", "ft", "delta", "accel",
+                                               "ang_accel")]
+
+
+class HistoryView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("nhist", "hist_key", "hist_delta", "nwhist", "whist_key", "whist_delta")]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("input_mode", ctypes.c_int), ("inlet_box", ctypes.c_double * 6),
+                ("radius", ctypes.c_double), ("density", ctypes.c_double), ("cor", ctypes.c_double),
+                ("mu", ctypes.c_double), ("young", ctypes.c_double), ("k", ctypes.c_double),
+                ("etaconst", ctypes.c_double), ("m", ctypes.c_double), ("g", ctypes.c_double * 3),
+                ("dt", ctypes.c_double), ("out_step", ctypes.c_int), ("steps", ctypes.c_int),
+                ("seed", ctypes.c_int), ("gpu_on", ctypes.c_int), ("brute_on", ctypes.c_int),
+                ("nwalls", ctypes.c_int)]
 
 
 _lib = None
@@ -146,6 +164,17 @@ def lib():
         L.gdem_batch_info.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int), P(ctypes.c_int64),
                                       P(ctypes.c_int64)]
         L.gdem_batch_timing.argtypes = [vp, P(ctypes.c_double)]
+        L.gdem_scene_set_history.argtypes = [vp, P(HistoryView)]
+        L.gdem_settings_load.argtypes = [ctypes.c_char_p, P(Settings)]
+        L.gdem_settings_wall.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.gdem_settings_particles.argtypes = [ctypes.c_char_p, vp, vp]
+        L.gdem_output_init.argtypes = [ctypes.c_char_p]
+        L.gdem_write_frame_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                                           ctypes.c_double]
+        L.gdem_write_pid_header.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.gdem_write_pid_row.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, vp, vp,
+                                         vp, vp, vp, ctypes.c_int]
+        L.gdem_write_initial_pos.argtypes = [ctypes.c_char_p, ctypes.c_int, vp, vp]
         _lib = L
     return _lib
 
@@ -359,12 +388,37 @@ class Scene:
         return d
 
     def step_record(self, cap=64):
+        """One step that records every contact (partner id or triangle id,
+        kind, fn, ft, overlap) and every particle's accelerations."""
         n = self.n
         d = dict(count=np.zeros(n, np.int32), key=np.empty((n, cap), np.int32),
                  kind=np.empty((n, cap), np.int32), fn=np.empty((n, cap, 3)), ft=np.empty((n, cap, 3)),
-                 delta=np.empty((n, cap)))
-        view = ContactView(cap, *[_ptr(d[f]) for f in ("count", "key", "kind", "fn", "ft", "delta")])
+                 delta=np.empty((n, cap)), accel=np.zeros((n, 3)), ang_accel=np.zeros((n, 3)))
+        view = ContactView(cap, *[_ptr(d[f]) for f in ("count", "key", "kind", "fn", "ft", "delta", "accel",
+                                                        "ang_accel")])
         self.ctx.check(lib().gdem_scene_step_record(self.h, ctypes.byref(view)), "gdem_scene_step_record")
+        return d
+
+    def set_history(self, hist):
+        """Replaces the contact histories (a dict as debug() returns: nhist,
+        hist_key, hist_delta, nwhist, whist_key, whist_delta)."""
+        n, c = self.n, self.caps
+        arrs = dict(nhist=np.ascontiguousarray(hist["nhist"], np.int32),
+                    hist_key=np.ascontiguousarray(hist["hist_key"], np.int32).reshape(n, c["hist"]),
+                    hist_delta=np.ascontiguousarray(hist["hist_delta"], np.float64).reshape(n, c["hist"], 3),
+                    nwhist=np.ascontiguousarray(hist["nwhist"], np.int32),
+                    whist_key=np.ascontiguousarray(hist["whist_key"], np.int32).reshape(n, c["whist"]),
+                    whist_delta=np.ascontiguousarray(hist["whist_delta"], np.float64).reshape(n, c["whist"], 3))
+        view = HistoryView(*[_ptr(arrs[f]) for f, _ in HistoryView._fields_])
+        self.ctx.check(lib().gdem_scene_set_history(self.h, ctypes.byref(view)), "gdem_scene_set_history")
+
+    def snapshot(self):
+        """SAVE_ON state: positions, velocities, spins, activity and the
+        contact histories, in original particle order."""
+        d = self.download()
+        dbg = self.debug()
+        for f in ("nhist", "hist_key", "hist_delta", "nwhist", "whist_key", "whist_delta"):
+            d[f] = dbg[f]
         return d
 
     def set_timing(self, every=1):
@@ -482,3 +536,74 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- demSettings.json and output files (host only) ---------------------------
+
+
+def _b(path):
+    return os.fsencode(path)
+
+
+def load_settings(path):
+    """demSettings.json as the C loader reads it (dict), plus the wall files."""
+    st = Settings()
+    rc = lib().gdem_settings_load(_b(path), ctypes.byref(st))
+    if rc:
+        raise GdemError(rc, f"gdem_settings_load({path})")
+    out = {f: (list(getattr(st, f)) if isinstance(getattr(st, f), ctypes.Array) else getattr(st, f))
+           for f, _ in Settings._fields_}
+    walls = []
+    buf = ctypes.create_string_buffer(4096)
+    for i in range(st.nwalls):
+        rc = lib().gdem_settings_wall(_b(path), i, buf, len(buf))
+        if rc:
+            raise GdemError(rc, f"gdem_settings_wall({path}, {i})")
+        walls.append(os.fsdecode(buf.value))
+    out["walls"] = walls
+    return out
+
+
+def settings_particles(path, n):
+    x = np.empty((n, 3))
+    v = np.empty((n, 3))
+    rc = lib().gdem_settings_particles(_b(path), _ptr(x), _ptr(v))
+    if rc:
+        raise GdemError(rc, f"gdem_settings_particles({path})")
+    return x, v
+
+
+def output_init(d):
+    rc = lib().gdem_output_init(_b(d))
+    if rc:
+        raise GdemError(rc, f"gdem_output_init({d})")
+
+
+def write_frame_bin(d, step, x, r, active, length_factor=1.0):
+    x = _f64(x, (-1, 3))
+    r = _f64(r).ravel()
+    a = np.ascontiguousarray(active, np.int32)
+    rc = lib().gdem_write_frame_bin(_b(d), int(step), len(r), _ptr(x), _ptr(r), _ptr(a), float(length_factor))
+    if rc:
+        raise GdemError(rc, "gdem_write_frame_bin")
+
+
+def write_pid_header(d, pid):
+    rc = lib().gdem_write_pid_header(_b(d), int(pid))
+    if rc:
+        raise GdemError(rc, "gdem_write_pid_header")
+
+
+def write_pid_row(d, step, dt, pid, x, v, a, angv, anga, active):
+    arrs = [_f64(q).ravel()[:3].copy() for q in (x, v, a, angv, anga)]
+    rc = lib().gdem_write_pid_row(_b(d), int(step), float(dt), int(pid), *[_ptr(q) for q in arrs], int(active))
+    if rc:
+        raise GdemError(rc, "gdem_write_pid_row")
+
+
+def write_initial_pos(d, x, r):
+    x = _f64(x, (-1, 3))
+    r = _f64(r).ravel()
+    rc = lib().gdem_write_initial_pos(_b(d), len(r), _ptr(x), _ptr(r))
+    if rc:
+        raise GdemError(rc, "gdem_write_initial_pos")

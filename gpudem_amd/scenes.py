@@ -8,6 +8,7 @@ Material follows the reference's set-up (main.cu:67, ParticleSystem.cu:202,
 unless a workload names its own.
 """
 import math
+import os
 
 import numpy as np
 
@@ -304,3 +305,25 @@ def served_stream(seed=5, nreq=100, rate=20.0, nmin=500, nmax=200000, smin=1000,
                          name=f"req{i}", **kw)
         out.append((t, spec, nsteps))
     return out
+
+
+def from_settings(path):
+    """The scene a demSettings.json describes, as main.cu builds it
+    (main.cu:57-227): material from particleTypes.particle0, walls from the
+    listed ASCII STL files, the inlet's particles, the bounding box of
+    calc_BoundingBoxLimits, cell 4 r, skin r, refresh r/2
+    (BoundingBox.cu:381-388) and dt from dtFactor/outputTiming.  Returns
+    (spec, settings dict)."""
+    from . import engine
+    st = engine.load_settings(path)
+    x, v = engine.settings_particles(path, st["n"])
+    verts, counts = read_ascii_stl(st["walls"]) if st["walls"] else (np.zeros((0, 9)), [])
+    r = np.full(st["n"], st["radius"])
+    bbox = bbox_rule(verts if len(verts) else None, st["inlet_box"])
+    params = dict(dt=st["dt"], mu=st["mu"], g=tuple(st["g"]), bbox=bbox, cell=4 * st["radius"],
+                  skin=st["radius"], refresh=0.5 * st["radius"])
+    spec = SceneSpec(os.path.basename(os.path.dirname(os.path.abspath(path))), x, r, verts, params,
+                     st["density"], 0.0, st["cor"], v=v, steps=st["steps"])
+    spec.tris_per_file = counts
+    spec.material = (np.full(st["n"], st["m"]), np.full(st["n"], st["k"]), np.full(st["n"], st["etaconst"]))
+    return spec, st

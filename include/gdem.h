@@ -135,7 +135,41 @@ typedef struct {
   double* fn;    /* [n*rec_cap*3] normal force */
   double* ft;    /* [n*rec_cap*3] tangential force */
   double* delta; /* [n*rec_cap] overlap */
+  double* accel;     /* [n*3] or NULL: linear acceleration of the step (f/m + g, k_integrate) */
+  double* ang_accel; /* [n*3] or NULL: angular acceleration of the step (mom/I) */
 } gdem_contact_view;
+
+/* Contact histories in original particle order, as gdem_scene_debug returns
+ * them (the SAVE_ON history members deltHis*, indHis*, numCont* of
+ * ParticleSystemMember_common.def:26-35). */
+typedef struct {
+  const int* nhist;          /* [n] */
+  const int* hist_key;       /* [n*hist_cap] partner original ids in slot order */
+  const double* hist_delta;  /* [n*hist_cap*3] */
+  const int* nwhist;         /* [n] */
+  const int* whist_key;      /* [n*whist_cap] triangle ids */
+  const double* whist_delta; /* [n*whist_cap*3] */
+} gdem_history_view;
+
+/* demSettings.json (main.cu:57-135, 227; ParticleSystem.cu:189-352). */
+#define GDEM_INLET_SHUFFLE 0
+#define GDEM_INLET_FILE 1
+typedef struct {
+  int n;                /* inlet.numParticle */
+  int input_mode;       /* GDEM_INLET_SHUFFLE or GDEM_INLET_FILE */
+  double inlet_box[6];  /* shuffle box, or the file's coordinate range */
+  double radius, density, cor, mu;
+  double young;         /* YoungModulus, 0 when the legacy stiffness "k" is given */
+  double k;             /* normal stiffness (ParticleSystem.cu:202 from YoungModulus, or "k") */
+  double etaconst;      /* ParticleSystem.cu:301 */
+  double m;             /* density*3.14*r^3*4/3 (main.cu:67) */
+  double g[3];
+  double dt;            /* 2 PI sqrt(m/k)/dtFactor, adjusted to divide outputTiming (main.cu:123-129) */
+  int out_step;         /* steps between outputs */
+  int steps;            /* (int)(endTime/dt) (main.cu:227) */
+  int seed, gpu_on, brute_on;
+  int nwalls;           /* walls.filepaths */
+} gdem_settings;
 
 /* ---- context ---------------------------------------------------------- */
 /* Replaces the implicit CUDA context + legacy stream of main.cu:206-245. */
@@ -187,6 +221,10 @@ void gdem_scene_destroy(gdem_scene* scene);
 int gdem_scene_debug(gdem_scene* scene, gdem_debug_view* view);
 /* Runs one step that also records every contact's forces. */
 int gdem_scene_step_record(gdem_scene* scene, gdem_contact_view* view);
+/* Replaces the scene's contact histories (snapshot restore: the SAVE_ON
+ * history members that copyToDevice carries, ParticleSystem.cu:82-89).
+ * GDEM_E_SLOT_OVERFLOW when a row exceeds the scene's capacities. */
+int gdem_scene_set_history(gdem_scene* scene, const gdem_history_view* view);
 /* Device time of the last gdem_scene_step call (ms) and the summed time of
  * the force-kernel launches it sampled, with the number sampled; measured
  * with HIP events on the context stream. */
@@ -249,6 +287,31 @@ int gdem_batch_info(gdem_batch* batch, int64_t* step, int* live_scenes, int64_t*
                     int64_t* bytes);
 /* Device time (ms) of the last gdem_batch_step call; waits for it. */
 int gdem_batch_timing(gdem_batch* batch, double* ms);
+
+/* ---- settings and output files (host only, no GPU) ------------------------ */
+/* Reads demSettings.json.  Keys match case-insensitively (as
+ * cJSON_GetObjectItem).  A missing "YoungModulus" falls back to the legacy
+ * "k"; a missing "bruteOn"/"gpuOn" means 0. */
+int gdem_settings_load(const char* path, gdem_settings* out);
+/* i-th walls.filepaths entry, relative paths resolved against the folder of
+ * the settings file. */
+int gdem_settings_wall(const char* path, int i, char* buf, int len);
+/* Initial positions and velocities [n*3]: the file inlet's CSV, or the
+ * shuffle inlet drawn exactly as initializeParticles (ParticleSystem.cu:
+ * 205-285: std::mt19937 seeded with 1, overlap rejection, 1000 trials). */
+int gdem_settings_particles(const char* path, double* x, double* v);
+/* solver_output_init (solver_output.cu:14-41) */
+int gdem_output_init(const char* dir);
+/* write_frame_bin (solver_output.cu:259-316): dir/result_%012d.bin */
+int gdem_write_frame_bin(const char* dir, int step, int n, const double* x, const double* r,
+                         const int* is_active, double length_factor);
+/* write_header_text (solver_output.cu:62-85): dir/pid<pid>.txt */
+int gdem_write_pid_header(const char* dir, int pid);
+/* write_single_text (solver_output.cu:112-172), unit factors */
+int gdem_write_pid_row(const char* dir, int step, double dt, int pid, const double x[3], const double v[3],
+                       const double a[3], const double angv[3], const double anga[3], int is_active);
+/* write_initialPos_csv (solver_output.cu:87-110): dir/initialPos.csv */
+int gdem_write_initial_pos(const char* dir, int n, const double* x, const double* r);
 
 /* ---- primitives exposed for parity tests ---------------------------------- */
 /* Stable LSD radix sort of (key, iota) on the device: the order
