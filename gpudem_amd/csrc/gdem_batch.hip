@@ -224,11 +224,24 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
   const int sid = __builtin_amdgcn_readfirstlane(B.wave_scene[__builtin_amdgcn_readfirstlane(s >> 6)]);
   uint32_t err = 0;
   bool refresh = false;
+  // this workgroup's positions, for in-tile neighbour reads; a neighbour
+  // belongs to its particle's scene, whose waves stage from the same buffer
+  __shared__ double4 tile_p[256];
+  bool running = false;
+  int q = 0;
+  double4 p = make_double4(0., 0., 0., -1.);
+  if (sid >= 0) {
+    const BScene& S = B.scenes[sid];
+    running = step >= S.start_step && step < S.end_step;
+    q = __builtin_amdgcn_readfirstlane(B.sctl[sid].q);
+    if (running) p = B.pos[(step & 1) ^ q][s];
+  }
+  tile_p[threadIdx.x] = p;
+  __syncthreads();
   if (sid >= 0) {
     const BScene& S = B.scenes[sid];
     SlotCtl& C = B.sctl[sid];
-    if (step >= S.start_step && step < S.end_step) {
-      const int q = __builtin_amdgcn_readfirstlane(C.q);
+    if (running) {
       const int cur = (step & 1) ^ q;
       StepArgs<H> A;
       A.pin = B.pos[cur];
@@ -262,7 +275,8 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
       }
       A.threshSq = S.threshSq;
       Rec<T> none{};
-      step_particle<T, H, false, PRow, WRow>(A, s, true, lds, none, err, refresh);
+      step_particle<T, H, false, PRow, WRow>(A, s, true, p, tile_p, s - (int)threadIdx.x, lds, none, err,
+                                             refresh);
       // a refresh at the scene's last step would only move its final state
       if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {
         C.fl[step & 1] = 1;

@@ -539,8 +539,14 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // walking its own list forms.  Wall candidates are distance-tested the same
 // way, then each particle handles its wall contacts in list order (vertex and
 // edge deduplication needs that order).
+//
+// p is the particle's position record, staged by the caller in
+// tile_p[threadIdx.x]; tile_p holds the records of slots [tile0, tile0 + 256),
+// this workgroup's particles, so neighbours inside the tile (most of them, in
+// Morton order) are read from LDS instead of memory.
 template <class T, class H, bool REC, class PRow, class WRow, class Lds>
-__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, Lds& lds, const Rec<T>& rec,
+__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, double4 p,
+                                              const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
                                               uint32_t& err, bool& refresh) {
   using H4 = typename Vec4<H>::type;
   const int tx = threadIdx.x;
@@ -550,9 +556,12 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const double4* __restrict__ pin = A.pin;
   const double4* __restrict__ vin = A.vin;
   const double4* __restrict__ win = A.win;
-  double4 p = make_double4(0., 0., 0., -1.), v = make_double4(0., 0., 0., 0.), w = v;
+  auto pos_of = [&](int j) -> double4 {
+    const unsigned t = (unsigned)(j - tile0);
+    return t < 256u ? tile_p[t] : pin[j];
+  };
+  double4 v = make_double4(0., 0., 0., 0.), w = v;
   if (valid) {
-    p = pin[s];
     v = vin[s];
     w = win[s];
   }
@@ -604,7 +613,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
 #pragma unroll
       for (int u = 0; u < 8; ++u) jv[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s].x : -1;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pin[jv[u]] : make_double4(0., 0., 0., -1.);
+      for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pos_of(jv[u]) : make_double4(0., 0., 0., -1.);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const double4 pj = cv[u];
@@ -644,7 +653,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         const int q = nth_bit(o_hits, g - o_first);
         e = A.nbr[(size_t)q * N + so];
         slot = row.find_in(wtx + o, so, o_n, e.y);
-        pj = pin[e.x];
+        pj = pos_of(e.x);
         vj = vin[e.x];
         wj = win[e.x];
         if (slot >= 0) d = ld_nt(row.gd + (size_t)slot * N + so);
