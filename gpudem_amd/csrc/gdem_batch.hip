@@ -228,54 +228,52 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
   // belongs to its particle's scene, whose waves stage from the same buffer
   __shared__ double4 tile_p[256];
   bool running = false;
-  int q = 0;
-  double4 p = make_double4(0., 0., 0., -1.);
+  StepArgs<H> A{};
   if (sid >= 0) {
     const BScene& S = B.scenes[sid];
     running = step >= S.start_step && step < S.end_step;
-    q = __builtin_amdgcn_readfirstlane(B.sctl[sid].q);
-    if (running) p = B.pos[(step & 1) ^ q][s];
+    const int q = __builtin_amdgcn_readfirstlane(B.sctl[sid].q);
+    const int cur = (step & 1) ^ q;
+    A.pin = B.pos[cur];
+    A.vin = B.vel[cur];
+    A.win = B.angv[cur];
+    A.pout = B.pos[cur ^ 1];
+    A.vout = B.vel[cur ^ 1];
+    A.wout = B.angv[cur ^ 1];
+    A.mat = B.mat[q];
+    A.hkey = B.hkey[q];
+    A.hd = B.hd[q];
+    A.nh = B.nh[q];
+    A.wkey = B.wkey[q];
+    A.wd = B.wd[q];
+    A.nw = B.nw[q];
+    A.ref = B.ref;
+    A.nbr = B.nbr;
+    A.nnb = B.nnb;
+    A.wl = B.wl;
+    A.nwl = B.nwl;
+    A.tri = B.tri;
+    A.N = B.cap;
+    A.hist_cap = B.hist_cap;
+    A.whist_cap = B.whist_cap;
+    A.dt = S.dt;
+    A.mu = S.mu;
+    for (int a = 0; a < 3; ++a) {
+      A.g[a] = S.g[a];
+      A.bmin[a] = S.bmin[a];
+      A.bmax[a] = S.bmax[a];
+    }
+    A.threshSq = S.threshSq;
   }
-  tile_p[threadIdx.x] = p;
+  const Own<H> own = load_own(A, s, running);
+  tile_p[threadIdx.x] = own.p;
   __syncthreads();
   if (sid >= 0) {
     const BScene& S = B.scenes[sid];
     SlotCtl& C = B.sctl[sid];
     if (running) {
-      const int cur = (step & 1) ^ q;
-      StepArgs<H> A;
-      A.pin = B.pos[cur];
-      A.vin = B.vel[cur];
-      A.win = B.angv[cur];
-      A.pout = B.pos[cur ^ 1];
-      A.vout = B.vel[cur ^ 1];
-      A.wout = B.angv[cur ^ 1];
-      A.mat = B.mat[q];
-      A.hkey = B.hkey[q];
-      A.hd = B.hd[q];
-      A.nh = B.nh[q];
-      A.wkey = B.wkey[q];
-      A.wd = B.wd[q];
-      A.nw = B.nw[q];
-      A.ref = B.ref;
-      A.nbr = B.nbr;
-      A.nnb = B.nnb;
-      A.wl = B.wl;
-      A.nwl = B.nwl;
-      A.tri = B.tri;
-      A.N = B.cap;
-      A.hist_cap = B.hist_cap;
-      A.whist_cap = B.whist_cap;
-      A.dt = S.dt;
-      A.mu = S.mu;
-      for (int a = 0; a < 3; ++a) {
-        A.g[a] = S.g[a];
-        A.bmin[a] = S.bmin[a];
-        A.bmax[a] = S.bmax[a];
-      }
-      A.threshSq = S.threshSq;
       Rec<T> none{};
-      step_particle<T, H, false, PRow, WRow>(A, s, true, p, tile_p, s - (int)threadIdx.x, lds, none, err,
+      step_particle<T, H, false, PRow, WRow>(A, s, true, own, tile_p, s - (int)threadIdx.x, lds, none, err,
                                              refresh);
       // a refresh at the scene's last step would only move its final state
       if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {

@@ -73,6 +73,11 @@ __device__ __forceinline__ double4 ld_nt(const double4* p) {
   const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
   return make_double4(v[0], v[1], v[2], v[3]);
 }
+__device__ __forceinline__ int2 ld_nt(const int2* p) {
+  typedef int v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return make_int2(v[0], v[1]);
+}
 __device__ __forceinline__ float4 ld_nt(const float4* p) {
   typedef float v4 __attribute__((ext_vector_type(4)));
   const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
@@ -117,26 +122,26 @@ struct TriHit {
   int hit;
 };
 
-__device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, double px, double py,
-                                               double pz, double Rsq) {
+template <class G>
+__device__ __forceinline__ TriHit tri_distance_g(const G& tg, double px, double py, double pz, double Rsq) {
   TriHit res;
   res.n = V3<double>{0., 0., 0.};
   res.dist = 1.e10;
   res.hit = -1;
-  const V3<double> nt{tg->n[0], tg->n[1], tg->n[2]};
+  const V3<double> nt{tg.n[0], tg.n[1], tg.n[2]};
   double dist = 0.;
   dist += nt.x * px;
   dist += nt.y * py;
   dist += nt.z * pz;
-  dist += tg->d;
+  dist += tg.d;
   double distsq = dist * dist;
   if (distsq > Rsq) return res;
   const V3<double> pos{px, py, pz};
-  const V3<double> P0{tg->p0[0], tg->p0[1], tg->p0[2]};
+  const V3<double> P0{tg.p0[0], tg.p0[1], tg.p0[2]};
   const V3<double> ap = vsub(pos, P0);
-  const double d00 = tg->d00, d01 = tg->d01, d11 = tg->d11, den = tg->denom;
-  const double d20 = ap.x * tg->e01[0] + ap.y * tg->e01[1] + ap.z * tg->e01[2];
-  const double d21 = ap.x * tg->e02[0] + ap.y * tg->e02[1] + ap.z * tg->e02[2];
+  const double d00 = tg.d00, d01 = tg.d01, d11 = tg.d11, den = tg.denom;
+  const double d20 = ap.x * tg.e01[0] + ap.y * tg.e01[1] + ap.z * tg.e01[2];
+  const double d21 = ap.x * tg.e02[0] + ap.y * tg.e02[1] + ap.z * tg.e02[2];
   const double v = (d11 * d20 - d01 * d21) * den;
   const double w = (d00 * d21 - d01 * d20) * den;
   const double u = 1.0 - v - w;
@@ -151,32 +156,32 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
   int hit;
   if (v <= 0. && w <= 0.0) {
     q = P0;
-    hit = tg->v0;
+    hit = tg.v0;
   } else if (u <= 0. && w <= 0.0) {
-    q = V3<double>{tg->p1[0], tg->p1[1], tg->p1[2]};
-    hit = tg->v1;
+    q = V3<double>{tg.p1[0], tg.p1[1], tg.p1[2]};
+    hit = tg.v1;
   } else if (u <= 0. && v <= 0.0) {
-    q = V3<double>{tg->p2[0], tg->p2[1], tg->p2[2]};
-    hit = tg->v2;
+    q = V3<double>{tg.p2[0], tg.p2[1], tg.p2[2]};
+    hit = tg.v2;
   } else if (u <= 0.0) {
-    const V3<double> P1{tg->p1[0], tg->p1[1], tg->p1[2]};
-    const V3<double> E{tg->e12[0], tg->e12[1], tg->e12[2]};
-    double t = dot(vsub(pos, P1), E) * tg->d22inv;
+    const V3<double> P1{tg.p1[0], tg.p1[1], tg.p1[2]};
+    const V3<double> E{tg.e12[0], tg.e12[1], tg.e12[2]};
+    double t = dot(vsub(pos, P1), E) * tg.d22inv;
     t = fmax(0., fmin(1.0, t));
     q = vadd(P1, scal(t, E));
-    hit = tg->e1;
+    hit = tg.e1;
   } else if (v <= 0.0) {
-    const V3<double> E{tg->e02[0], tg->e02[1], tg->e02[2]};
-    double t = d21 * tg->d11inv;
+    const V3<double> E{tg.e02[0], tg.e02[1], tg.e02[2]};
+    double t = d21 * tg.d11inv;
     t = fmax(0., fmin(1.0, t));
     q = vadd(P0, scal(t, E));
-    hit = tg->e2;
+    hit = tg.e2;
   } else {
-    const V3<double> E{tg->e01[0], tg->e01[1], tg->e01[2]};
-    double t = d20 * tg->d00inv;
+    const V3<double> E{tg.e01[0], tg.e01[1], tg.e01[2]};
+    double t = d20 * tg.d00inv;
     t = fmax(0., fmin(1.0, t));
     q = vadd(P0, scal(t, E));
-    hit = tg->e0;
+    hit = tg.e0;
   }
   res.n = vsub(pos, q);
   distsq = dot(res.n, res.n);
@@ -186,6 +191,12 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
   }
   res.hit = hit;
   return res;
+}
+
+// Reads the triangle's fields as the tests need them (the plane test first).
+__device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, double px, double py,
+                                               double pz, double Rsq) {
+  return tri_distance_g(*tg, px, py, pz, Rsq);
 }
 
 // ---------------------------------------------------------------------------
@@ -207,25 +218,32 @@ struct HistLds {
   uint64_t marks;  // slots in contact this step
   int* k;          // this thread's LDS column: k[q * 256]
   int* k0;         // thread 0's column; thread t's column is k0 + t
-  // ok: this lane's particle is active (other lanes load nothing of theirs)
-  __device__ __forceinline__ void load(int nh, bool ok) {
+  // ok: this lane's particle is active (other lanes load nothing of theirs).
+  // fetch issues the key loads into registers and keep stores them in LDS,
+  // so the loads of several rows can share one memory round trip.
+  __device__ __forceinline__ void fetch(int nh, bool ok, int (&kk)[CAP]) {
     n = nh;
     marks = 0;
-    // Every key load is issued before the first LDS store, so one memory
-    // round trip serves the whole row.  The wave loads as many slots as its
-    // longest row (never more than the capacity); lanes without a particle
-    // read slot row 0 of particle 0 instead, so every address is inside the
-    // [cap][N] array.  Slots past a lane's own row are not stored.
+    // The wave loads as many slots as its longest row (never more than the
+    // capacity); lanes without a particle read slot row 0 of particle 0
+    // instead, so every address is inside the [cap][N] array.  Slots past a
+    // lane's own row are not stored.
     int wmax = ok ? nh : 0;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
     wmax = min(__builtin_amdgcn_readfirstlane(wmax), min(cap, CAP));
     const int* src = gkey + (ok ? s : 0);
-    int kk[CAP];
 #pragma unroll
     for (int q = 0; q < CAP; ++q) kk[q] = q < wmax ? ld_nt(src + (size_t)q * N) : 0;
+  }
+  __device__ __forceinline__ void keep(const int (&kk)[CAP]) {
 #pragma unroll
     for (int q = 0; q < CAP; ++q)
-      if (q < nh) k[q * 256] = kk[q];
+      if (q < n) k[q * 256] = kk[q];
+  }
+  __device__ __forceinline__ void load(int nh, bool ok) {
+    int kk[CAP];
+    fetch(nh, ok, kk);
+    keep(kk);
   }
   __device__ __forceinline__ int find(int key) const {
     for (int q = 0; q < n; ++q)
@@ -409,11 +427,16 @@ constexpr int step_waves() {
   return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
 }
 
+// Touching pairs of a particle whose list entries the screening pass keeps in
+// LDS for the force pass (later ones are read again from the list).
+constexpr int HIT_STASH = 4;
+
 template <int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
   int ve[8][256];
+  int2 hit[HIT_STASH][256];
 };
 
 template <class Row>
@@ -422,6 +445,22 @@ template <class H, int CAP>
 __device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col0, int tx) {
   r.k0 = col0;
   r.k = col0 + tx;
+}
+
+// Loads the pair and wall history rows of a particle; LDS rows issue all
+// their key loads before the first LDS store.
+template <class A, class B>
+__device__ __forceinline__ void load_rows(A& a, int na, B& b, int nb, bool ok) {
+  a.load(na, ok);
+  b.load(nb, ok);
+}
+template <class H, int CA, int CB>
+__device__ __forceinline__ void load_rows(HistLds<H, CA>& a, int na, HistLds<H, CB>& b, int nb, bool ok) {
+  int ka[CA], kb[CB];
+  a.fetch(na, ok, ka);
+  b.fetch(nb, ok, kb);
+  a.keep(ka);
+  b.keep(kb);
 }
 
 template <class Row>
@@ -465,6 +504,39 @@ struct StepArgs {
   double g[3], bmin[3], bmax[3];
   double threshSq;
 };
+
+// A particle's own records: state, material, reference position, list and
+// history lengths.  All loads are independent, so they share one memory round
+// trip; lanes without a particle load nothing.
+template <class H>
+struct Own {
+  double4 p, v, w, rf;
+  typename Vec4<H>::type mt;
+  int nnb, nwl, nh0, nw0;
+};
+
+template <class H>
+__device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool valid) {
+  Own<H> o;
+  o.p = make_double4(0., 0., 0., -1.);
+  o.v = make_double4(0., 0., 0., 0.);
+  o.w = o.v;
+  o.rf = o.p;
+  o.mt = {};
+  o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
+  if (valid) {
+    o.p = A.pin[s];
+    o.v = A.vin[s];
+    o.w = A.win[s];
+    o.mt = ld_nt(A.mat + s);
+    o.rf = ld_nt(A.ref + s);
+    o.nnb = ld_nt(A.nnb + s);
+    o.nwl = ld_nt(A.nwl + s);
+    o.nh0 = ld_nt(A.nh + s);
+    o.nw0 = ld_nt(A.nw + s);
+  }
+  return o;
+}
 
 // ---- wave-level helpers (64 lanes) ------------------------------------------
 __device__ __forceinline__ int wave_incl_sum(int v) {
@@ -540,12 +612,13 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // way, then each particle handles its wall contacts in list order (vertex and
 // edge deduplication needs that order).
 //
-// p is the particle's position record, staged by the caller in
-// tile_p[threadIdx.x]; tile_p holds the records of slots [tile0, tile0 + 256),
-// this workgroup's particles, so neighbours inside the tile (most of them, in
-// Morton order) are read from LDS instead of memory.
+// own holds the particle's records (load_own, issued by the caller before it
+// stages own.p in tile_p[threadIdx.x]); tile_p holds the position records of
+// slots [tile0, tile0 + 256), this workgroup's particles, so neighbours inside
+// the tile (most of them, in Morton order) are read from LDS instead of
+// memory.
 template <class T, class H, bool REC, class PRow, class WRow, class Lds>
-__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, double4 p,
+__device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, const Own<H>& own,
                                               const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
                                               uint32_t& err, bool& refresh) {
   using H4 = typename Vec4<H>::type;
@@ -556,32 +629,27 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const double4* __restrict__ pin = A.pin;
   const double4* __restrict__ vin = A.vin;
   const double4* __restrict__ win = A.win;
+  // The LDS read is marked non-temporal (no effect on an LDS access) so the
+  // compiler cannot fold it and the memory read into one flat load of a
+  // selected address: flat loads go through the vector-memory path even for
+  // LDS addresses, and the memory read keeps its normal cache policy.
   auto pos_of = [&](int j) -> double4 {
     const unsigned t = (unsigned)(j - tile0);
-    return t < 256u ? tile_p[t] : pin[j];
+    double4 r = ld_nt(tile_p + (t & 255u));
+    if (t >= 256u) r = pin[j];
+    return r;
   };
-  double4 v = make_double4(0., 0., 0., 0.), w = v;
-  if (valid) {
-    v = vin[s];
-    w = win[s];
-  }
+  const double4 p = own.p, v = own.v, w = own.w;
   const bool act = valid && p.w >= 0.;
   if (valid && !act) {
     st_nt(p, A.pout + s);
     st_nt(v, A.vout + s);
     st_nt(w, A.wout + s);
   }
-  H4 mt{};
-  int nnb = 0, nwl = 0, nh0 = 0, nw0 = 0;
-  double4 rf = p;
-  if (act) {
-    mt = ld_nt(A.mat + s);
-    nnb = ld_nt(A.nnb + s);
-    nwl = ld_nt(A.nwl + s);
-    rf = ld_nt(A.ref + s);  // position at the last list build
-    nh0 = ld_nt(A.nh + s);
-    nw0 = ld_nt(A.nw + s);
-  }
+  const H4 mt = own.mt;
+  const double4 rf = own.rf;  // position at the last list build
+  const int nnb = act ? own.nnb : 0, nwl = act ? own.nwl : 0;
+  const int nh0 = act ? own.nh0 : 0, nw0 = act ? own.nw0 : 0;
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
   const double invmoi = (double)mt.w;
   const double ri = p.w, invm = v.w;
@@ -593,27 +661,51 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   int nrec = 0;
   const int self_id = (int)w.w;
 
+  // history rows of pairs and walls
+  PRow prow;
+  bind_lds(prow, &lds.pkey[0][0], tx);
+  prow.gkey = A.hkey;
+  prow.gd = A.hd;
+  prow.s = s;
+  prow.N = N;
+  prow.cap = A.hist_cap;
+  WRow wrow;
+  bind_lds(wrow, &lds.wkey[0][0], tx);
+  wrow.gkey = A.wkey;
+  wrow.gd = A.wd;
+  wrow.s = s;
+  wrow.N = N;
+  wrow.cap = A.whist_cap;
+  load_rows(prow, nh0, wrow, nw0, act);
+  // the keys are read from other lanes' columns below
+  __builtin_amdgcn_wave_barrier();
+  // The wave's wall candidates are spread over its lanes, 64 at a time; the
+  // first 64 are fetched now so the load overlaps the pair pass.
+  const int w_incl = wave_incl_sum(nwl);
+  const int w_total = __builtin_amdgcn_readfirstlane(__shfl(w_incl, 63));
+  int w_t0 = 0;
+  {
+    // shuffles run with every lane active
+    const int o = owner_lane(w_incl, lane);
+    const int o_first = __shfl(w_incl - nwl, o), so = __shfl(s, o);
+    if (lane < w_total) w_t0 = ld_nt(A.wl + (size_t)(lane - o_first) * N + so);
+  }
+
   // ---- particle pairs -------------------------------------------------------
   {
-    PRow row;
-    bind_lds(row, &lds.pkey[0][0], tx);
-    row.gkey = A.hkey;
-    row.gd = A.hd;
-    row.s = s;
-    row.N = N;
-    row.cap = A.hist_cap;
-    row.load(nh0, act);
-    // the keys are read from other lanes' columns below
-    __builtin_amdgcn_wave_barrier();
+    PRow& row = prow;
     // screening: which candidates overlap (FP64 test of the reference)
+    // The first HIT_STASH touching entries go to this thread's LDS column.
     uint64_t hits = 0;
+    int nhit = 0;
+    int2* stash = &lds.hit[0][tx];
     for (int q0 = 0; q0 < nnb; q0 += 8) {
-      int jv[8];
+      int2 ev[8];
       double4 cv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) jv[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s].x : -1;
+      for (int u = 0; u < 8; ++u) ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : make_int2(-1, 0);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) cv[u] = jv[u] >= 0 ? pos_of(jv[u]) : make_double4(0., 0., 0., -1.);
+      for (int u = 0; u < 8; ++u) cv[u] = ev[u].x >= 0 ? pos_of(ev[u].x) : make_double4(0., 0., 0., -1.);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const double4 pj = cv[u];
@@ -621,9 +713,15 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
         const double distsq = dx * dx + dy * dy + dz * dz;
         const double R = ri + pj.w;
-        if (distsq < R * R) hits |= (uint64_t)1 << (q0 + u);
+        if (distsq < R * R) {
+          hits |= (uint64_t)1 << (q0 + u);
+          if (nhit < HIT_STASH) stash[nhit * 256] = ev[u];
+          ++nhit;
+        }
       }
     }
+    // the entries are read from other lanes' columns below
+    __builtin_amdgcn_wave_barrier();
     // the touching pairs, spread over the lanes
     const int cnt = __popcll(hits);
     const int incl = wave_incl_sum(cnt);
@@ -650,8 +748,10 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       double4 pj = make_double4(0., 0., 0., 0.), vj = pj, wj = pj;
       H4 d{};
       if (has) {
-        const int q = nth_bit(o_hits, g - o_first);
-        e = A.nbr[(size_t)q * N + so];
+        const int r = g - o_first;  // rank among the owner's touching pairs
+        // (the LDS read is marked non-temporal so it stays an LDS instruction)
+        e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
+        if (r >= HIT_STASH) e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
         slot = row.find_in(wtx + o, so, o_n, e.y);
         pj = pos_of(e.x);
         vj = vin[e.x];
@@ -757,22 +857,13 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
 
   // ---- walls ----------------------------------------------------------------
   {
-    WRow row;
-    bind_lds(row, &lds.wkey[0][0], tx);
-    row.gkey = A.wkey;
-    row.gd = A.wd;
-    row.s = s;
-    row.N = N;
-    row.cap = A.whist_cap;
-    row.load(nw0, act);
+    WRow& row = wrow;
     int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
     int nve = 0;
     const double rsq = ri * ri;
     const T eta = etac * sqrtm;
     // distance tests of the wave's wall candidates, spread over the lanes
-    const int incl = wave_incl_sum(nwl);
-    const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
-    const int first = incl - nwl;
+    const int incl = w_incl, total = w_total, first = incl - nwl;
     for (int base = 0; base < total; base += 64) {
       const int g = base + lane;
       const bool has = g < total;
@@ -781,13 +872,18 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int so = __shfl(s, o);
       const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
       const double orsq = __shfl(rsq, o);
-      int t = 0;
+      const int o_n = __shfl(nw0, o);
+      int t = 0, slot = -1;
+      H4 d{};
       TriHit th;
       th.n = V3<double>{0., 0., 0.};
       th.dist = 1.e10;
       th.hit = -1;
       if (has) {
-        t = ld_nt(A.wl + (size_t)(g - o_first) * N + so);
+        // candidate, its history slot and displacement, and its distance
+        t = base == 0 ? w_t0 : ld_nt(A.wl + (size_t)(g - o_first) * N + so);
+        slot = row.find_in(wtx + o, so, o_n, t);
+        if (slot >= 0) d = ld_nt(row.gd + (size_t)slot * N + so);
         th = tri_distance(A.tri + t, opx, opy, opz, orsq);
       }
       // every particle handles its candidates of this round in list order
@@ -802,6 +898,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         h.dist = __shfl(th.dist, src);
         h.hit = __shfl(th.hit, src);
         const int tt = __shfl(t, src);
+        const int hs = __shfl(slot, src);
+        const V3<T> old{__shfl((T)d.x, src), __shfl((T)d.y, src), __shfl((T)d.z, src)};
         if (!mine || !(h.dist < ri)) continue;
         if (h.hit != -1) {
           bool dup = false;
@@ -832,7 +930,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         f.x += fn.x;
         f.y += fn.y;
         f.z += fn.z;
-        const V3<T> ft = tangential<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, tt, fn, vt, n, eta, k, mu, dtT);
+        const V3<T> ft = tangential_at<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, tt, hs,
+                                          hs >= 0 ? old : V3<T>{T(0), T(0), T(0)}, fn, vt, n, eta, k, mu, dtT);
         f.x += ft.x;
         f.y += ft.y;
         f.z += ft.z;

@@ -58,12 +58,12 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   // this workgroup's positions, for in-tile neighbour reads
   __shared__ double4 tile_p[256];
   const bool valid = s < P.N;
-  const double4 p = valid ? A.pin[s] : make_double4(0., 0., 0., -1.);
-  tile_p[threadIdx.x] = p;
+  const Own<H> own = load_own(A, s, valid);
+  tile_p[threadIdx.x] = own.p;
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
-  step_particle<T, H, REC, PRow, WRow>(A, s, valid, p, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
+  step_particle<T, H, REC, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
   if (err) atomicOr((unsigned int*)&B.ctl[CTL_ERR], err);
   if (last_block(&B.ctl[CTL_DONE_STEP])) {
