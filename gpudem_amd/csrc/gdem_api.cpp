@@ -19,8 +19,8 @@ struct gdem_scene {
   gdem::SimParams P{};
   int precision = GDEM_FP64;
   int graph_steps = 0;
-  bool coop = false;   // rebuild as one cooperative kernel (else the launch chain)
-  bool chain = false;  // launch chain requested (graph capture or profiling)
+  bool coop = false;   // rebuild as one grid-barrier kernel (else the launch chain)
+  bool chain = false;  // launch chain requested (per-phase profiling)
   int64_t steps = 0;
   int rebuild_base = 0;
   gdem::MeshHost mesh;
@@ -80,7 +80,9 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
   HIPCHK(c, A.alloc(&B.cell_tmp, N));
   HIPCHK(c, A.alloc(&B.cell_of, N));
   HIPCHK(c, A.alloc(&B.cell_tab, ncell));
-  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)256 * sc->P.rs_blocks + 256));
+  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)RS_DIGITS * sc->P.rs_blocks + RS_DIGITS));
+  HIPCHK(c, A.alloc(&B.gbar, GBAR_WORDS));
+  HIPCHK(c, hipMemsetAsync(B.gbar, 0, GBAR_WORDS * sizeof(uint32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(B.cell_tab, 0xff, ncell * sizeof(int2), c->stream));
   HIPCHK(c, hipMemsetAsync(B.cell_of, 0xff, N * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(B.nh[0], 0, N * sizeof(int), c->stream));
@@ -182,18 +184,21 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   const int N = sc->P.N;
   size_t ncell = (size_t)sc->P.size[0] * sc->P.size[1] * sc->P.size[2];
   Bufs<H>& B = bufs<H>(sc);
-  // radix-sort / cooperative grid: one workgroup per chunk of >= 256 keys
+  // radix-sort / rebuild grid: one workgroup per chunk of >= 256 keys
   SimParams& P = sc->P;
   const int want = (N + 255) / 256;
-  // The cooperative grid stays well below the co-residency the occupancy
-  // query reports (half of it, at most 1024 workgroups): every workgroup must
-  // be resident at once for the grid barriers, and the rebuild phases are
-  // memory-bound loops that do not need more.
+  // The rebuild kernel's grid stays well below the co-residency the
+  // occupancy query reports (half of it, at most 1024 workgroups): every
+  // workgroup must be resident at once for its grid barriers (a normal
+  // launch of a grid that fits is resident once the preceding kernel has
+  // drained; the barrier's bounded wait turns anything else into an error),
+  // and the rebuild phases are memory-bound loops that do not need more.
   const int limit = sc->chain ? 0 : std::min(1024, coop_grid_limit<H>() / 2);
   sc->coop = limit > 0;
   P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want))
                          : std::max(1, std::min(1024, (N + 2047) / 2048));
   P.rs_chunk = ((N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
+  P.list_blocks = list_grid<H>(N);
   int rc = alloc_bufs<H>(sc, B, ncell);
   if (rc) return rc;
   using H4 = typename Vec4<H>::type;
@@ -546,7 +551,7 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
     return fail(c, GDEM_E_ARG, "params: slot capacities exceed 64 (pairs/histories) or 48 (walls)");
   sc->precision = pp->precision;
   sc->graph_steps = pp->graph_steps > 0 ? pp->graph_steps : 0;
-  sc->chain = pp->graph_steps != 0;
+  sc->chain = pp->graph_steps < 0;
   // every active particle must lie in the box (the cell search assumes it)
   for (int i = 0; i < pd->n; ++i) {
     if (pd->is_active && pd->is_active[i] != 1) continue;
@@ -625,6 +630,7 @@ int gdem_scene_status(gdem_scene* sc, gdem_stats* st) {
                    GDEM_ERR_WHIST_OVERFLOW | GDEM_ERR_VE_OVERFLOW))
     return fail(sc->ctx, GDEM_E_SLOT_OVERFLOW, "slot overflow (raise the capacities in gdem_params)");
   if (o->errors & GDEM_ERR_NAN) return fail(sc->ctx, GDEM_E_NAN, "non-finite particle state");
+  if (o->errors & GDEM_ERR_BARRIER) return fail(sc->ctx, GDEM_E_HIP, "list rebuild: grid barrier timed out");
   return GDEM_OK;
 }
 
@@ -682,7 +688,7 @@ int gdem_radix_sort(gdem_ctx* c, int n, const uint32_t* keys, uint32_t* keys_out
   HIPCHK(c, tmp.alloc(&k1, n));
   HIPCHK(c, tmp.alloc(&i0, n));
   HIPCHK(c, tmp.alloc(&i1, n));
-  HIPCHK(c, tmp.alloc(&hist, (size_t)256 * blocks));
+  HIPCHK(c, tmp.alloc(&hist, (size_t)RS_DIGITS * blocks + RS_DIGITS));
   HIPCHK(c, tmp.alloc(&ctl, CTL_WORDS));
   std::vector<int> iota(n);
   for (int i = 0; i < n; ++i) iota[i] = i;
@@ -691,8 +697,8 @@ int gdem_radix_sort(gdem_ctx* c, int n, const uint32_t* keys, uint32_t* keys_out
   launch_radix_sort(n, blocks, chunk, k0, i0, k1, i1, hist, ctl, false, c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(keys_out, k0, (size_t)n * 4, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(order_out, i0, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(keys_out, RS_FINAL ? k1 : k0, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(order_out, RS_FINAL ? i1 : i0, (size_t)n * 4, hipMemcpyDeviceToHost));
   return GDEM_OK;
 }
 

@@ -101,14 +101,10 @@ __device__ __forceinline__ void st_nt(float4 a, float4* p) {
 // once per workgroup); results reach the next kernel through the kernel
 // boundary.
 __device__ __forceinline__ bool last_block(int* counter) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  return is_last && threadIdx.x == 0;
+  __syncthreads();  // every thread of the workgroup is past its reads
+  if (threadIdx.x != 0) return false;
+  const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return prev == (int)gridDim.x - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1036,107 +1032,147 @@ struct ListArgs {
 // all active j != i in the 27 surrounding cells with d.d < (r_i+r_j+skin)^2,
 // ordered by original id.  Also records the reference position.
 constexpr int NB_MAX = 64;
-constexpr int NB_REG = 32;  // capacities up to this keep the list in registers
 
-// Inserts (id, q) into the first n entries of kp/kq, kept ascending in id.
-// Every index is a compile-time constant, so the arrays stay in registers.
-template <int R>
-__device__ __forceinline__ void insert_sorted(int (&kp)[R], int (&kq)[R], int n, int id, int q) {
-  int at = 0;
-#pragma unroll
-  for (int k = 0; k < R; ++k) at += (k < n && kp[k] < id) ? 1 : 0;
-#pragma unroll
-  for (int k = R - 1; k > 0; --k) {
-    const bool up = k > at && k <= n;
-    kp[k] = up ? kp[k - 1] : kp[k];
-    kq[k] = up ? kq[k - 1] : kq[k];
-  }
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    kp[k] = k == at ? id : kp[k];
-    kq[k] = k == at ? q : kq[k];
-  }
-}
+// Candidates are met slab by slab: each z-slab of 3 x 3 cells has its nine
+// cell ranges loaded at once, then its particles are tested PL_BATCH at a
+// time with their positions and ids loaded together (a few memory round trips
+// per slab instead of one per cell and per particle).  visit(q, id) is called
+// for every candidate in range.
+constexpr int PL_BATCH = 8;
 
-// R: size of the per-thread candidate arrays (>= the capacity used).
-template <int R>
-__device__ __forceinline__ void pairlist_core(const ListArgs& L, int s) {
-  const int N = L.N;
+template <class Visit>
+__device__ __forceinline__ void pair_candidates(const ListArgs& L, int s, const double4& p, Visit&& visit) {
   const double4* __restrict__ pos = L.pos;
   const int* __restrict__ pid = L.pid;
-  const double4 p = pos[s];
   const int c = L.cell_of[s] - L.cell_base;
   const int sx = L.size[0], sy = L.size[1], sz = L.size[2];
   const int cx = c % sx;
   const int cy = (c / sx) % sy;
   const int cz = c / (sx * sy);
-  int cand[R];
-  int cpid[R];
-  int n = 0;
-  bool ovf = false;
-  const int cap = L.nb_cap < R ? L.nb_cap : R;
   for (int dz = -1; dz <= 1; ++dz) {
     const int Z = cz + dz;
     if (Z < 0 || Z >= sz) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int Y = cy + dy;
-      if (Y < 0 || Y >= sy) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int X = cx + dx;
-        if (X < 0 || X >= sx) continue;
-        const int cc = L.cell_base + (sy * Z + Y) * sx + X;
-        const int2 ce = L.cell_tab[cc];
-        if (ce.x < 0) continue;
-        const int st = ce.x, en = ce.y;
-        for (int q = st; q < en; ++q) {
-          if (q == s) continue;
-          const double4 pj = pos[q];
-          const double ddx = p.x - pj.x, ddy = p.y - pj.y, ddz = p.z - pj.z;
-          const double distsq = ddx * ddx + ddy * ddy + ddz * ddz;
-          const double R2 = p.w + pj.w + L.skin;
-          if (distsq < R2 * R2) {
-            if (n >= cap) {
-              ovf = true;
-              continue;
-            }
-            const int id = pid[q];
-            if constexpr (R <= NB_REG) {
-              insert_sorted<R>(cpid, cand, n, id, q);
-            } else {
-              int at = n;
-              while (at > 0 && cpid[at - 1] > id) {
-                cpid[at] = cpid[at - 1];
-                cand[at] = cand[at - 1];
-                --at;
-              }
-              cpid[at] = id;
-              cand[at] = q;
-            }
-            ++n;
-          }
+    int st[9], cnt[9];
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int Y = cy + k / 3 - 1, X = cx + k % 3 - 1;
+      int2 ce = make_int2(-1, -1);
+      if (Y >= 0 && Y < sy && X >= 0 && X < sx) ce = L.cell_tab[L.cell_base + (sy * Z + Y) * sx + X];
+      st[k] = ce.x;
+      cnt[k] = ce.x < 0 ? 0 : ce.y - ce.x;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) total += cnt[k];
+    for (int t0 = 0; t0 < total; t0 += PL_BATCH) {
+      int qv[PL_BATCH];
+#pragma unroll
+      for (int u = 0; u < PL_BATCH; ++u) {
+        // slot of the slab's (t0 + u)-th particle
+        int rem = t0 + u, q = -1;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          if (q < 0 && rem < cnt[k]) q = st[k] + rem;
+          if (q < 0) rem -= cnt[k];
         }
+        qv[u] = (t0 + u < total && q != s) ? q : -1;
+      }
+      double4 pj[PL_BATCH];
+      int idj[PL_BATCH];
+#pragma unroll
+      for (int u = 0; u < PL_BATCH; ++u) {
+        pj[u] = qv[u] >= 0 ? pos[qv[u]] : make_double4(0., 0., 0., -1.);
+        idj[u] = qv[u] >= 0 ? pid[qv[u]] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < PL_BATCH; ++u) {
+        if (qv[u] < 0) continue;
+        const double ddx = p.x - pj[u].x, ddy = p.y - pj[u].y, ddz = p.z - pj[u].z;
+        const double distsq = ddx * ddx + ddy * ddy + ddz * ddz;
+        const double R2 = p.w + pj[u].w + L.skin;
+        if (distsq < R2 * R2) visit(qv[u], idj[u]);
       }
     }
   }
-  if constexpr (R <= NB_REG) {
-#pragma unroll
-    for (int q = 0; q < R; ++q)
-      if (q < n) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
-  } else {
-    for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+}
+
+// Lists up to PL_LDS_CAP long are gathered in the thread's LDS column
+// (col[k * PL_BLOCK]), one store per accepted candidate, and sorted there once
+// at the end; longer capacities use private arrays.
+constexpr int PL_BLOCK = 128;   // threads per workgroup of the list kernels
+constexpr int PL_LDS_CAP = 32;
+
+struct PairLds {
+  int2 e[PL_LDS_CAP][PL_BLOCK];
+};
+
+__device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col) {
+  const int N = L.N;
+  const double4 p = L.pos[s];
+  const int cap = L.nb_cap;
+  int n = 0;
+  bool ovf = false;
+  pair_candidates(L, s, p, [&](int q, int id) {
+    if (n >= cap) {
+      ovf = true;
+      return;
+    }
+    col[n * PL_BLOCK] = make_int2(q, id);
+    ++n;
+  });
+  // ids are distinct, so the order is total
+  for (int i = 1; i < n; ++i) {
+    const int2 x = col[i * PL_BLOCK];
+    int j = i - 1;
+    for (; j >= 0; --j) {
+      const int2 y = col[j * PL_BLOCK];
+      if (y.y < x.y) break;
+      col[(j + 1) * PL_BLOCK] = y;
+    }
+    col[(j + 1) * PL_BLOCK] = x;
   }
+  for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * PL_BLOCK];
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
 }
 
-__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active) {
+__device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
+  const int N = L.N;
+  const double4 p = L.pos[s];
+  const int cap = L.nb_cap < NB_MAX ? L.nb_cap : NB_MAX;
+  int cand[NB_MAX];
+  int cpid[NB_MAX];
+  int n = 0;
+  bool ovf = false;
+  pair_candidates(L, s, p, [&](int q, int id) {
+    if (n >= cap) {
+      ovf = true;
+      return;
+    }
+    int at = n;
+    while (at > 0 && cpid[at - 1] > id) {
+      cpid[at] = cpid[at - 1];
+      cand[at] = cand[at - 1];
+      --at;
+    }
+    cpid[at] = id;
+    cand[at] = q;
+    ++n;
+  });
+  for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+  L.nnb[s] = n;
+  if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
+}
+
+// lds: the workgroup's PairLds (threads of a PL_BLOCK-thread workgroup)
+__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active, PairLds& lds) {
   L.ref[s] = L.pos[s];
   if (!active) {
     L.nnb[s] = 0;
     return;
   }
-  if (L.nb_cap <= NB_REG) pairlist_core<NB_REG>(L, s);
-  else pairlist_core<NB_MAX>(L, s);
+  if (L.nb_cap <= PL_LDS_CAP) pairlist_lds(L, s, &lds.e[0][threadIdx.x]);
+  else pairlist_arrays(L, s);
 }
 
 // Wall candidates by BVH traversal (k_update_neighborlist_wall,
@@ -1171,20 +1207,35 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
   if (L.wall_table) {
     // static per-cell table: ascending triangle ids already, so candidates
     // go straight to their slots
+    // (entries and their boxes are loaded WL_BATCH at a time)
+    constexpr int WL_BATCH = 4;
     const int c = L.cell_of[s];
-    const int b1 = L.wc_off[c + 1];
-    for (int q = L.wc_off[c]; q < b1; ++q) {
-      const int t = L.wc_tri[q];
-      const double* bx = L.tbox + (size_t)t * 6;
-      if (!consider(t, bx, bx + 3)) continue;
-      const TriHit h = tri_distance(L.tri + t, p.x, p.y, p.z, Rsq);
-      if (h.dist < p.w + L.skin) {
-        if (n >= cap) {
-          ovf = true;
-          continue;
+    const int b0 = L.wc_off[c], b1 = L.wc_off[c + 1];
+    for (int q0 = b0; q0 < b1; q0 += WL_BATCH) {
+      int tv[WL_BATCH];
+#pragma unroll
+      for (int u = 0; u < WL_BATCH; ++u) tv[u] = q0 + u < b1 ? L.wc_tri[q0 + u] : -1;
+      bool near[WL_BATCH];
+#pragma unroll
+      for (int u = 0; u < WL_BATCH; ++u) {
+        near[u] = false;
+        if (tv[u] >= 0) {
+          const double* bx = L.tbox + (size_t)tv[u] * 6;
+          near[u] = consider(tv[u], bx, bx + 3);
         }
-        L.wl[(size_t)n * N + s] = t;
-        ++n;
+      }
+      for (int u = 0; u < WL_BATCH; ++u) {
+        if (!near[u]) continue;
+        const int t = tv[u];
+        const TriHit h = tri_distance(L.tri + t, p.x, p.y, p.z, Rsq);
+        if (h.dist < p.w + L.skin) {
+          if (n >= cap) {
+            ovf = true;
+            continue;
+          }
+          L.wl[(size_t)n * N + s] = t;
+          ++n;
+        }
       }
     }
     L.nwl[s] = n;

@@ -10,8 +10,22 @@
 #define GDEM_ERR_WHIST_OVERFLOW_BIT 8u
 #define GDEM_ERR_VE_OVERFLOW_BIT 16u
 #define GDEM_ERR_NAN_BIT 32u
+#define GDEM_ERR_BARRIER_BIT 64u
 
 namespace gdem {
+
+// Stable LSD radix sort of 32-bit keys: three passes of 11-bit digits; the
+// sorted keys and their source indices end in buffer RS_FINAL of the
+// ping-pong pair.  The count table holds RS_DIGITS entries per workgroup plus
+// RS_DIGITS digit totals.
+constexpr int RS_BITS = 11;
+constexpr int RS_DIGITS = 1 << RS_BITS;
+constexpr int RS_PASSES = 3;
+constexpr int RS_FINAL = RS_PASSES & 1;
+// Two-level grid barrier of the rebuild kernel: BAR_SUB arrival
+// counters, a top counter and a generation word, each on its own 128 B line.
+constexpr int BAR_SUB = 32;
+constexpr int GBAR_WORDS = (BAR_SUB + 2) * 32;
 
 struct RecDev {
   int cap;
@@ -55,14 +69,18 @@ template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st);
 template <class T, class H>
 void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st);
-// coop: one cooperative kernel (grid = P.rs_blocks, co-resident); otherwise
-// the launch chain, which can be captured into a hipGraph.
+// coop: one grid-barrier kernel (grid = P.rs_blocks, co-resident); otherwise
+// the chain of per-phase kernels.  Both can be captured into a hipGraph.
 template <class H>
 hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipStream_t st);
-// Largest co-resident grid of the cooperative rebuild kernel (0 if the
-// device cannot launch cooperatively).
+// Largest co-resident grid of the grid-barrier rebuild kernel (0 if the
+// occupancy query fails).
 template <class H>
 int coop_grid_limit();
+// Grid of the pair-list kernel for n slots: no more workgroups than can be
+// resident at once (the kernel strides over the slots).
+template <class H>
+int list_grid(int n);
 template <class H>
 void launch_download(const SimParams& P, const Bufs<H>& B, double* x, double* v, double* w, int* a,
                      hipStream_t st);

@@ -7,7 +7,6 @@
 // ([slot][N]) so lane i of a wave reads consecutive words.  A list rebuild is
 // decided on the device (refresh flag) and every rebuild kernel exits at once
 // when the flag is clear, so a step needs no host synchronisation.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "gdem_device.h"
@@ -74,7 +73,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
 
 // ---------------------------------------------------------------------------
 // list rebuild (each phase is a per-element device function used both by the
-// launch chain and by the single cooperative rebuild kernel)
+// launch chain and by the single grid-barrier rebuild kernel)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool rebuild_due(const int* ctl) {
   return __builtin_amdgcn_readfirstlane(ctl[CTL_REFRESH]) != 0;
@@ -84,7 +83,7 @@ __device__ __forceinline__ bool rebuild_due(const int* ctl) {
 // Also clears the table entry of the cell this particle occupied at the last
 // rebuild, so the new table holds exactly the occupied cells.
 template <class H>
-__device__ __forceinline__ void keys_one(const SimParams& P, const Bufs<H>& B, int s, int cur) {
+__device__ __forceinline__ uint32_t keys_one(const SimParams& P, const Bufs<H>& B, int s, int cur) {
   const int old = B.cell_of[s];
   if (old >= 0) B.cell_tab[old] = make_int2(-1, -1);
   const double4 p = B.pos[cur][s];
@@ -100,36 +99,92 @@ __device__ __forceinline__ void keys_one(const SimParams& P, const Bufs<H>& B, i
   B.key[0][s] = key;
   B.idx[0][s] = s;
   B.cell_tmp[s] = cid;
+  return key;
 }
 
-// ---- stable LSD radix sort, 8-bit digits ----------------------------------
+// ---- stable LSD radix sort, 11-bit digits ---------------------------------
 // Workgroup b owns the contiguous chunk [b*chunk, (b+1)*chunk); per-(digit,
 // chunk) counts are scanned digit-major so equal digits keep chunk order, and
 // inside a chunk keys are ranked in arrival order with wave ballots.
 constexpr int RS_THREADS = 256;
+constexpr int RS_PER = RS_DIGITS / RS_THREADS;  // digits per thread
+constexpr int RS_WAVES = RS_THREADS / 64;
 
-__device__ __forceinline__ void rs_count_block(int n, int chunk, int shift, const uint32_t* __restrict__ keys,
-                                               uint32_t* __restrict__ hist, int b, int nb) {
-  __shared__ uint32_t cnt[256];
-  cnt[threadIdx.x] = 0;
+__device__ __forceinline__ uint32_t rs_digit(uint32_t k, int pass) {
+  return (k >> (RS_BITS * pass)) & (uint32_t)(RS_DIGITS - 1);
+}
+
+// cnt (LDS, RS_DIGITS words) -> hist[d * nb + b]
+__device__ __forceinline__ void rs_clear(uint32_t* cnt) {
+#pragma unroll
+  for (int q = 0; q < RS_PER; ++q) cnt[q * RS_THREADS + threadIdx.x] = 0;
   __syncthreads();
-  const int beg = b * chunk;
-  const int end = min(n, beg + chunk);
-  for (int i = beg + threadIdx.x; i < end; i += RS_THREADS) atomicAdd(&cnt[(keys[i] >> shift) & 0xffu], 1u);
+}
+__device__ __forceinline__ void rs_store_counts(const uint32_t* cnt, uint32_t* __restrict__ hist, int b, int nb) {
   __syncthreads();
-  hist[(size_t)threadIdx.x * nb + b] = cnt[threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < RS_PER; ++q) {
+    const int d = q * RS_THREADS + threadIdx.x;
+    hist[(size_t)d * nb + b] = cnt[d];
+  }
   __syncthreads();
 }
 
-// Exclusive scan of m counters in place by one workgroup of RS_THREADS
-// threads; returns the total.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* __restrict__ v, int m) {
+__device__ __forceinline__ void rs_count_block(int n, int chunk, int pass, const uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ hist, int b, int nb, uint32_t* cnt) {
+  rs_clear(cnt);
+  const int beg = b * chunk;
+  const int end = min(n, beg + chunk);
+  for (int i = beg + threadIdx.x; i < end; i += RS_THREADS) atomicAdd(&cnt[rs_digit(keys[i], pass)], 1u);
+  rs_store_counts(cnt, hist, b, nb);
+}
+
+// Exclusive scan, in place, of each digit's counts over the nb chunks
+// (hist[d * nb + j]), with the digit totals to dtot[d].  Workgroup b of gs
+// takes digits b, b + gs, ...: one thread per digit when nb is small, one
+// wave per digit otherwise (no workgroup barriers either way).
+__device__ __forceinline__ void rs_scan_digits(uint32_t* __restrict__ hist, uint32_t* __restrict__ dtot, int nb,
+                                               int b, int gs) {
+  if (nb <= 64) {
+    for (int d = b * RS_THREADS + threadIdx.x; d < RS_DIGITS; d += gs * RS_THREADS) {
+      uint32_t* h = hist + (size_t)d * nb;
+      uint32_t run = 0;
+      for (int j = 0; j < nb; ++j) {
+        const uint32_t c = h[j];
+        h[j] = run;
+        run += c;
+      }
+      dtot[d] = run;
+    }
+  } else {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int d = b * RS_WAVES + wave; d < RS_DIGITS; d += gs * RS_WAVES) {
+      uint32_t* h = hist + (size_t)d * nb;
+      int run = 0;
+      for (int j0 = 0; j0 < nb; j0 += 64) {
+        const int j = j0 + lane;
+        const int c = j < nb ? (int)h[j] : 0;
+        const int incl = wave_incl_sum(c);
+        if (j < nb) h[j] = (uint32_t)(run + incl - c);
+        run += __shfl(incl, 63);
+      }
+      if (lane == 0) dtot[d] = (uint32_t)run;
+    }
+  }
+}
+
+// Scatter bases of workgroup b: base[d] (LDS) = keys of smaller digits in the
+// whole array + keys of digit d in earlier chunks (the scanned hist entry).
+__device__ __forceinline__ void rs_bases(const uint32_t* __restrict__ hist, const uint32_t* __restrict__ dtot,
+                                         int b, int nb, uint32_t* base) {
   __shared__ uint32_t part[RS_THREADS];
-  const int per = (m + RS_THREADS - 1) / RS_THREADS;
-  const int beg = threadIdx.x * per;
-  const int end = min(m, beg + per);
+  uint32_t v[RS_PER];
   uint32_t sum = 0;
-  for (int i = beg; i < end; ++i) sum += v[i];
+#pragma unroll
+  for (int q = 0; q < RS_PER; ++q) {
+    v[q] = dtot[threadIdx.x * RS_PER + q];
+    sum += v[q];
+  }
   part[threadIdx.x] = sum;
   __syncthreads();
   for (int off = 1; off < RS_THREADS; off <<= 1) {
@@ -139,62 +194,77 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* __restrict__ 
     __syncthreads();
   }
   uint32_t run = part[threadIdx.x] - sum;
-  for (int i = beg; i < end; ++i) {
-    const uint32_t c = v[i];
-    v[i] = run;
-    run += c;
+#pragma unroll
+  for (int q = 0; q < RS_PER; ++q) {
+    const int d = threadIdx.x * RS_PER + q;
+    base[d] = run + hist[(size_t)d * nb + b];
+    run += v[q];
   }
-  const uint32_t total = part[RS_THREADS - 1];
   __syncthreads();
-  return total;
 }
 
-// Stable scatter of workgroup b's chunk.  base[d] must hold the global
-// position of the first key with digit d in this chunk.
-__device__ __forceinline__ void rs_scatter_block(int n, int chunk, int shift, const uint32_t* __restrict__ kin,
+// Stable scatter of workgroup b's chunk from the bases in base (LDS).  Each
+// round ranks 256 keys: within a wave by ballots over the digit bits, across
+// waves through per-wave digit counts in wcnt (LDS, RS_WAVES x RS_DIGITS
+// 16-bit words: round tag << 8 | count), so the counts are cleared only when
+// the 8-bit tag wraps.
+__device__ __forceinline__ void rs_clear_wcnt(uint16_t* wcnt) {
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RS_WAVES * RS_PER; ++q) wcnt[q * RS_THREADS + threadIdx.x] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void rs_scatter_block(int n, int chunk, int pass, const uint32_t* __restrict__ kin,
                                                  const int* __restrict__ iin, uint32_t* __restrict__ kout,
-                                                 int* __restrict__ iout, uint32_t* base, int b) {
-  constexpr int NW = RS_THREADS / 64;
-  __shared__ uint32_t wcnt[NW][256];
+                                                 int* __restrict__ iout, uint32_t* base, uint16_t* wcnt, int b) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  for (int w = 0; w < NW; ++w) wcnt[w][tid] = 0;
-  __syncthreads();
   const int beg = b * chunk;
   const int end = min(n, beg + chunk);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t round = 255;
   for (int r0 = beg; r0 < end; r0 += RS_THREADS) {
+    // tags run 1..255; cleared words (tag 0) never match
+    if (++round == 256) {
+      rs_clear_wcnt(wcnt);
+      round = 1;
+    }
     const int i = r0 + tid;
     const bool valid = i < end;
     const uint32_t k = valid ? kin[i] : 0u;
     const int idx = valid ? iin[i] : 0;
-    const uint32_t d = (k >> shift) & 0xffu;
+    const uint32_t d = rs_digit(k, pass);
     uint64_t match = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
+    for (int bit = 0; bit < RS_BITS; ++bit) {
       const uint64_t bal = __ballot((d >> bit) & 1u);
       match &= ((d >> bit) & 1u) ? bal : ~bal;
     }
     const int rank = __popcll(match & lt);
-    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(match);
+    const uint32_t cw = (uint32_t)__popcll(match);
+    if (valid && rank == 0) wcnt[wave * RS_DIGITS + d] = (uint16_t)((round << 8) | cw);
     __syncthreads();
     uint32_t before = 0;
-    if (valid)
-      for (int w = 0; w < wave; ++w) before += wcnt[w][d];
-    const uint32_t pos = base[d] + before + (uint32_t)rank;
-    __syncthreads();
-    {
-      uint32_t tot = 0;
-      for (int w = 0; w < NW; ++w) {
-        tot += wcnt[w][tid];
-        wcnt[w][tid] = 0;
-      }
-      base[tid] += tot;
-    }
+    bool last_wave = true;  // no later wave holds digit d this round
+    uint32_t bd = 0;
     if (valid) {
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) {
+        const uint32_t e = wcnt[w * RS_DIGITS + d];  // cw <= 64 fits the low byte
+        const bool live = (e >> 8) == round;
+        if (w < wave && live) before += e & 0xffu;
+        if (w > wave && live) last_wave = false;
+      }
+      bd = base[d];
+    }
+    __syncthreads();  // every base read before the updates below
+    if (valid) {
+      const uint32_t pos = bd + before + (uint32_t)rank;
       kout[pos] = k;
       iout[pos] = idx;
+      if (rank == 0 && last_wave) base[d] = bd + before + cw;
     }
     __syncthreads();
   }
@@ -205,7 +275,7 @@ __device__ __forceinline__ void rs_scatter_block(int n, int chunk, int shift, co
 template <class H>
 __device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B, int s, int cur, int hc) {
   const int N = P.N;
-  const int src = B.idx[0][s];
+  const int src = B.idx[RS_FINAL][s];
   B.pos[cur ^ 1][s] = B.pos[cur][src];
   B.vel[cur ^ 1][s] = B.vel[cur][src];
   B.angv[cur ^ 1][s] = B.angv[cur][src];
@@ -231,7 +301,7 @@ __device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B
 template <class H>
 __device__ __forceinline__ void cells_one(const SimParams& P, const Bufs<H>& B, int s) {
   const int N = P.N;
-  const uint32_t* key = B.key[0];
+  const uint32_t* key = B.key[RS_FINAL];
   const uint32_t k = key[s];
   if (k == 0xffffffffu) {
     if (s == 0) B.ctl[CTL_NACT] = 0;
@@ -253,22 +323,26 @@ __global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
 }
 
 template <bool GATED>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_count(int n, int chunk, int shift,
+__global__ __launch_bounds__(RS_THREADS) void k_rs_count(int n, int chunk, int pass,
                                                          const uint32_t* __restrict__ keys,
                                                          uint32_t* __restrict__ hist, const int* ctl) {
   if (GATED && !rebuild_due(ctl)) return;
-  rs_count_block(n, chunk, shift, keys, hist, blockIdx.x, gridDim.x);
+  __shared__ uint32_t cnt[RS_DIGITS];
+  rs_count_block(n, chunk, pass, keys, hist, blockIdx.x, gridDim.x, cnt);
 }
 
-// Exclusive scan of the whole digit-major count table by one workgroup.
+// The digits' counts over the nb chunks, scanned; the totals go to
+// hist[RS_DIGITS * nb + d].
+constexpr int RS_SCAN_BLOCKS = RS_DIGITS / RS_WAVES;
+
 template <bool GATED>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(int m, uint32_t* __restrict__ hist, const int* ctl) {
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(int nb, uint32_t* __restrict__ hist, const int* ctl) {
   if (GATED && !rebuild_due(ctl)) return;
-  block_exclusive_scan(hist, m);
+  rs_scan_digits(hist, hist + (size_t)RS_DIGITS * nb, nb, blockIdx.x, gridDim.x);
 }
 
 template <bool GATED>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int shift,
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int pass,
                                                            const uint32_t* __restrict__ kin,
                                                            const int* __restrict__ iin,
                                                            uint32_t* __restrict__ kout,
@@ -276,10 +350,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int
                                                            const uint32_t* __restrict__ hist,
                                                            const int* ctl) {
   if (GATED && !rebuild_due(ctl)) return;
-  __shared__ uint32_t base[256];
-  base[threadIdx.x] = hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
-  __syncthreads();
-  rs_scatter_block(n, chunk, shift, kin, iin, kout, iout, base, blockIdx.x);
+  __shared__ uint32_t base[RS_DIGITS];
+  __shared__ uint16_t wcnt[RS_WAVES * RS_DIGITS];
+  const int nb = gridDim.x;
+  rs_bases(hist, hist + (size_t)RS_DIGITS * nb, blockIdx.x, nb, base);
+  rs_scatter_block(n, chunk, pass, kin, iin, kout, iout, base, wcnt, blockIdx.x);
 }
 
 template <class H>
@@ -296,49 +371,61 @@ __global__ __launch_bounds__(256) void k_permute(SimParams P, Bufs<H> B) {
   }
 }
 
+// Arguments of the list builders for state buffer cur and material/id
+// buffer hc.
+template <class H>
+__device__ __forceinline__ ListArgs list_args(const SimParams& P, const Bufs<H>& B, int cur, int hc) {
+  ListArgs L;
+  L.pos = B.pos[cur];
+  L.pid = B.pid[hc];
+  L.cell_of = B.cell_of;
+  L.cell_tab = B.cell_tab;
+  L.nbr = B.nbr;
+  L.nnb = B.nnb;
+  L.ref = B.ref;
+  L.wl = B.wl;
+  L.nwl = B.nwl;
+  L.tri = B.tri;
+  L.bvh = B.bvh;
+  L.tbox = B.tbox;
+  L.wc_off = B.wc_off;
+  L.wc_tri = B.wc_tri;
+  L.err = (unsigned int*)&B.ctl[CTL_ERR];
+  L.N = P.N;
+  for (int a = 0; a < 3; ++a) L.size[a] = P.size[a];
+  L.cell_base = 0;
+  L.skin = P.skin;
+  L.nb_cap = P.nb_cap;
+  L.wall_cap = P.wall_cap;
+  L.ntri = P.ntri;
+  L.wall_table = P.wall_table;
+  return L;
+}
+
+// Cell table, and the wall list of each slot (it needs only the slot's own
+// permuted record and the static triangle tables).
 template <class H>
 __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < P.N) cells_one(P, B, s);
+  if (s < P.N) {
+    cells_one(P, B, s);
+    const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
+    walllist_one(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
+  }
 }
 
-// Pair and wall lists of the sorted particles, at full occupancy (both the
-// chain and the cooperative rebuild end here).  The last workgroup clears the
-// refresh flag, which ends the rebuild.
+// Pair lists of the sorted particles, at full occupancy (both the chain and
+// the grid-barrier rebuild end here).  The last workgroup clears the refresh
+// flag, which ends the rebuild.
 template <class H>
-__global__ __launch_bounds__(256) void k_lists(SimParams P, Bufs<H> B) {
+__global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < P.N) {
-    const int cur = B.ctl[CTL_CUR], hc = B.ctl[CTL_HCUR], nact = B.ctl[CTL_NACT];
-    ListArgs L;
-    L.pos = B.pos[cur];
-    L.pid = B.pid[hc];
-    L.cell_of = B.cell_of;
-    L.cell_tab = B.cell_tab;
-    L.nbr = B.nbr;
-    L.nnb = B.nnb;
-    L.ref = B.ref;
-    L.wl = B.wl;
-    L.nwl = B.nwl;
-    L.tri = B.tri;
-    L.bvh = B.bvh;
-    L.tbox = B.tbox;
-    L.wc_off = B.wc_off;
-    L.wc_tri = B.wc_tri;
-    L.err = (unsigned int*)&B.ctl[CTL_ERR];
-    L.N = P.N;
-    for (int a = 0; a < 3; ++a) L.size[a] = P.size[a];
-    L.cell_base = 0;
-    L.skin = P.skin;
-    L.nb_cap = P.nb_cap;
-    L.wall_cap = P.wall_cap;
-    L.ntri = P.ntri;
-    L.wall_table = P.wall_table;
-    pairlist_one(L, s, s < nact);
-    walllist_one(L, s, s < nact);
-  }
+  __shared__ PairLds lds;
+  const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
+  const int nact = B.ctl[CTL_NACT];
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.N; s += gridDim.x * blockDim.x)
+    pairlist_one(L, s, s < nact, lds);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -346,17 +433,62 @@ __global__ __launch_bounds__(256) void k_lists(SimParams P, Bufs<H> B) {
   }
 }
 
-// ---- the global rebuild phases in one cooperative kernel -------------------
-// Keys, the four radix passes, the permutation and the cell table need grid
-// barriers; they run in one cooperative kernel launched every step with a
-// co-resident grid, which returns at once (before any barrier) when the
-// refresh flag is clear.  Every workgroup reads the control words before the
+// ---- the global rebuild phases in one grid-barrier kernel ------------------
+// Two-level generation barrier over a co-resident grid: workgroups arrive on
+// BAR_SUB counters (b % BAR_SUB), the last arrival of each arrives on the top
+// counter, and the last of those resets it and advances the generation word,
+// which the others poll.  A single counter for every workgroup serialises the
+// arrivals (about 0.1 us each on MI355X).  Each workgroup releases its writes
+// before arriving and acquires after the generation moves; the counters are
+// reset before the generation word is released, so the next barrier finds
+// them at zero.  The wait is bounded: a barrier that does not complete within
+// about a second sets GDEM_ERR_BARRIER and lets the workgroup go on (the
+// rebuild's results are then invalid and the error is reported).
+constexpr int GBAR_SPIN = 1 << 20;
+
+__device__ __forceinline__ void grid_barrier(uint32_t* gbar, int* ctl) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int nsub = G < BAR_SUB ? G : BAR_SUB;
+    const int sub = b % nsub;
+    const int members = (G - sub + nsub - 1) / nsub;
+    uint32_t* cnt = gbar + sub * 32;
+    uint32_t* top = gbar + BAR_SUB * 32;
+    uint32_t* gen = gbar + (BAR_SUB + 1) * 32;
+    const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == (uint32_t)members - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == (uint32_t)nsub - 1) {
+        __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    for (int it = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0; ++it) {
+      if (it > GBAR_SPIN) {
+        atomicOr((unsigned int*)&ctl[CTL_ERR], GDEM_ERR_BARRIER_BIT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// Keys, the three radix passes, the permutation and the cell table need grid
+// barriers; they run in one kernel launched every step with a co-resident
+// grid, which returns at once (before any barrier) when the refresh flag is
+// clear.  Every workgroup reads the control words before the
 // first barrier; workgroup 0 flips the buffers at the end.  The lists follow
 // in k_lists.
 template <class H>
-__global__ __launch_bounds__(256) void k_rebuild_coop(SimParams P, Bufs<H> B) {
+__global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   const int cur = B.ctl[CTL_CUR];
   const int hc = B.ctl[CTL_HCUR];
   const int N = P.N;
@@ -365,41 +497,41 @@ __global__ __launch_bounds__(256) void k_rebuild_coop(SimParams P, Bufs<H> B) {
   const int stride = G * blockDim.x;
   const int first = b * blockDim.x + threadIdx.x;
   uint32_t* hist = B.rs_hist;
-  uint32_t* dtot = B.rs_hist + (size_t)256 * G;
-  for (int s = first; s < N; s += stride) keys_one(P, B, s, cur);
-  grid.sync();
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t dpre[256];
-  for (int pass = 0; pass < 4; ++pass) {
-    const int a = pass & 1;
-    const int shift = 8 * pass;
-    rs_count_block(N, P.rs_chunk, shift, B.key[a], hist, b, G);
-    grid.sync();
-    for (int d = b; d < 256; d += G) {
-      const uint32_t tot = block_exclusive_scan(hist + (size_t)d * G, G);
-      if (threadIdx.x == 0) dtot[d] = tot;
-    }
-    grid.sync();
-    // global base of digit d = totals of smaller digits + this chunk's prefix
-    dpre[threadIdx.x] = dtot[threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t run = 0;
-      for (int d = 0; d < 256; ++d) {
-        const uint32_t c = dpre[d];
-        dpre[d] = run;
-        run += c;
-      }
-    }
-    __syncthreads();
-    base[threadIdx.x] = dpre[threadIdx.x] + hist[(size_t)threadIdx.x * G + b];
-    __syncthreads();
-    rs_scatter_block(N, P.rs_chunk, shift, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1], base, b);
-    grid.sync();
+  uint32_t* dtot = B.rs_hist + (size_t)RS_DIGITS * G;
+  __shared__ uint32_t dig[RS_DIGITS];  // digit counts, then scatter bases
+  __shared__ uint16_t wcnt[RS_WAVES * RS_DIGITS];
+  // keys of this workgroup's chunk, counted for the first pass as they are made
+  rs_clear(dig);
+  {
+    const int beg = b * P.rs_chunk;
+    const int end = min(N, beg + P.rs_chunk);
+    for (int s = beg + threadIdx.x; s < end; s += RS_THREADS) atomicAdd(&dig[rs_digit(keys_one(P, B, s, cur), 0)], 1u);
   }
+  rs_store_counts(dig, hist, b, G);
+  grid_barrier(B.gbar, B.ctl);
+  for (int pass = 0; pass < RS_PASSES; ++pass) {
+    const int a = pass & 1;
+    if (pass > 0) {
+      rs_count_block(N, P.rs_chunk, pass, B.key[a], hist, b, G, dig);
+      grid_barrier(B.gbar, B.ctl);
+    }
+    rs_scan_digits(hist, dtot, G, b, G);
+    grid_barrier(B.gbar, B.ctl);
+    rs_bases(hist, dtot, b, G, dig);
+    rs_scatter_block(N, P.rs_chunk, pass, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1], dig, wcnt, b);
+    grid_barrier(B.gbar, B.ctl);
+  }
+  // The cell table needs the sorted keys (complete at the last barrier) and
+  // the cell of each slot, which this thread's own permute step wrote; so
+  // does the wall list, with the slot's permuted position.
   for (int s = first; s < N; s += stride) permute_one(P, B, s, cur, hc);
-  grid.sync();
-  for (int s = first; s < N; s += stride) cells_one(P, B, s);
+  {
+    const ListArgs L = list_args(P, B, cur ^ 1, hc ^ 1);
+    for (int s = first; s < N; s += stride) {
+      cells_one(P, B, s);
+      walllist_one(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
+    }
+  }
   if (b == 0 && threadIdx.x == 0) {
     B.ctl[CTL_CUR] = cur ^ 1;
     B.ctl[CTL_HCUR] = hc ^ 1;
@@ -596,18 +728,29 @@ void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st) {
   uint32_t* kk[2] = {k0, k1};
   int* ii[2] = {i0, i1};
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = 0; pass < RS_PASSES; ++pass) {
     const int a = pass & 1, b = a ^ 1;
     if (gated) {
-      k_rs_count<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], hist, ctl);
-      k_rs_scan<true><<<1, RS_THREADS, 0, st>>>(256 * blocks, hist, ctl);
-      k_rs_scatter<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
+      k_rs_count<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], hist, ctl);
+      k_rs_scan<true><<<RS_SCAN_BLOCKS, RS_THREADS, 0, st>>>(blocks, hist, ctl);
+      k_rs_scatter<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
     } else {
-      k_rs_count<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], hist, ctl);
-      k_rs_scan<false><<<1, RS_THREADS, 0, st>>>(256 * blocks, hist, ctl);
-      k_rs_scatter<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, 8 * pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
+      k_rs_count<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], hist, ctl);
+      k_rs_scan<false><<<RS_SCAN_BLOCKS, RS_THREADS, 0, st>>>(blocks, hist, ctl);
+      k_rs_scatter<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
     }
   }
+}
+
+template <class H>
+int list_grid(int n) {
+  const int want = blocks_for(n, PL_BLOCK);
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return want;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lists<H>, PL_BLOCK, 0) != hipSuccess || per_cu < 1)
+    return want;
+  return std::max(1, std::min(want, per_cu * prop.multiProcessorCount));
 }
 
 template <class H>
@@ -615,21 +758,17 @@ int coop_grid_limit() {
   int dev = 0, per_cu = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  if (!prop.cooperativeLaunch) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_coop<H>, 256, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_grid<H>, 256, 0) != hipSuccess) return 0;
   return per_cu * prop.multiProcessorCount;
 }
 
 template <class H>
 hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipStream_t st) {
   if (coop) {
-    SimParams p = P;
-    Bufs<H> b = B;
-    void* args[] = {&p, &b};
-    const hipError_t e =
-        hipLaunchCooperativeKernel((const void*)k_rebuild_coop<H>, dim3(P.rs_blocks), dim3(256), args, 0, st);
-    if (e != hipSuccess) return e;
-    k_lists<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B);
+    // A normal launch: hipLaunchCooperativeKernel adds about 11 us of idle
+    // time on each side of the kernel, and the grid is sized to be resident.
+    k_rebuild_grid<H><<<P.rs_blocks, 256, 0, st>>>(P, B);
+    k_lists<H><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
     return hipGetLastError();
   }
   const int g = blocks_for(P.N, 256);
@@ -638,7 +777,7 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
                     B.ctl, true, st);
   k_permute<H><<<g, 256, 0, st>>>(P, B);
   k_cells<H><<<g, 256, 0, st>>>(P, B);
-  k_lists<H><<<g, 256, 0, st>>>(P, B);
+  k_lists<H><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
   return hipGetLastError();
 }
 
@@ -681,6 +820,8 @@ template hipError_t launch_rebuild<double>(const SimParams&, const Bufs<double>&
 template hipError_t launch_rebuild<float>(const SimParams&, const Bufs<float>&, bool, hipStream_t);
 template int coop_grid_limit<double>();
 template int coop_grid_limit<float>();
+template int list_grid<double>(int);
+template int list_grid<float>(int);
 template void launch_download<double>(const SimParams&, const Bufs<double>&, double*, double*, double*, int*, hipStream_t);
 template void launch_download<float>(const SimParams&, const Bufs<float>&, double*, double*, double*, int*, hipStream_t);
 template void launch_debug<double>(const SimParams&, const Bufs<double>&, const DebugDev&, hipStream_t);

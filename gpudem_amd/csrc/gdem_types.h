@@ -46,6 +46,7 @@ struct SimParams {
   int nb_cap, wall_cap, hist_cap, whist_cap;
   int ntri, nnode;
   int rs_blocks, rs_chunk;  // radix sort grid and per-block chunk
+  int list_blocks;          // grid of the pair-list kernel (co-resident workgroups, grid-stride)
   int wall_table;           // 1: wall candidates from the static per-cell table, 0: BVH traversal
 };
 
@@ -101,7 +102,8 @@ struct Bufs {
   int* cell_tmp;     // cell id per particle before the permutation
   int* cell_of;      // cell id per particle after it
   int2* cell_tab;    // [ncell] (first, end) of each occupied cell, (-1, -1) otherwise
-  uint32_t* rs_hist; // [256 * rs_blocks]
+  uint32_t* rs_hist; // radix counts [RS_DIGITS * rs_blocks] + digit totals [RS_DIGITS]
+  uint32_t* gbar;    // grid barrier words of the rebuild kernel [GBAR_WORDS], zeroed at creation
   const TriGeo* tri;
   const BvhNode* bvh;
   const double* tbox;   // per-triangle box [ntri][6]: lo xyz, hi xyz

@@ -38,6 +38,7 @@ extern "C" {
 #define GDEM_ERR_WHIST_OVERFLOW 8u
 #define GDEM_ERR_VE_OVERFLOW 16u
 #define GDEM_ERR_NAN 32u
+#define GDEM_ERR_BARRIER 64u /* a grid barrier of the list rebuild timed out */
 
 typedef struct gdem_ctx gdem_ctx;
 typedef struct gdem_scene gdem_scene;
@@ -82,9 +83,10 @@ typedef struct {
   int wall_cap;     /* wall candidates per particle (0 = 24) */
   int hist_cap;     /* pair contact history slots (0 = 16) */
   int whist_cap;    /* wall contact history slots (0 = 8) */
-  int graph_steps;  /* 0: direct launches, list rebuild as one cooperative kernel;
-                       > 0: launch-chain rebuild captured in hipGraphs of this many steps;
-                       < 0: launch-chain rebuild, direct launches (per-phase profiling) */
+  int graph_steps;  /* 0: direct launches; > 0: steps captured per hipGraph (the list
+                       rebuild is one grid-barrier kernel either way);
+                       < 0: list rebuild as a chain of per-phase kernels, direct launches
+                       (per-phase profiling) */
   int wall_broadphase; /* 0: static per-cell triangle table when it fits, else BVH;
                           1: BVH traversal; 2: per-cell table (error if it does not fit) */
 } gdem_params;
