@@ -427,13 +427,25 @@ constexpr int step_waves() {
 // LDS for the force pass (later ones are read again from the list).
 constexpr int HIT_STASH = 4;
 
+// The hit stash is used only in the pair pass and the vertex/edge features
+// only in the wall pass, so they share their LDS: a thread's eight feature
+// words are the two halves of its own four stash entries (ve_slot), so no
+// thread's features overlap another thread's stash, whatever pass the other
+// waves of the workgroup are in.
+constexpr int VE_MAX = 2 * HIT_STASH;
+
 template <int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
-  int ve[8][256];
   int2 hit[HIT_STASH][256];
 };
+
+// feature word u (< VE_MAX) of thread tx
+template <class Lds>
+__device__ __forceinline__ int* ve_slot(Lds& lds, int tx, int u) {
+  return reinterpret_cast<int*>(&lds.hit[u >> 1][tx]) + (u & 1);
+}
 
 template <class Row>
 __device__ __forceinline__ void bind_lds(Row&, int*, int) {}
@@ -854,7 +866,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   // ---- walls ----------------------------------------------------------------
   {
     WRow& row = wrow;
-    int* ve = &lds.ve[0][tx];  // vertex/edge features touched this step, ve[u * 256]
+    // (every lane's pair pass is done before the wall pass writes the union)
+    __builtin_amdgcn_wave_barrier();
+    // vertex/edge features touched this step: *ve_slot(lds, tx, u), u < VE_MAX
     int nve = 0;
     const double rsq = ri * ri;
     const T eta = etac * sqrtm;
@@ -899,14 +913,14 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         if (!mine || !(h.dist < ri)) continue;
         if (h.hit != -1) {
           bool dup = false;
-          const int seen = nve < 8 ? nve : 8;
+          const int seen = nve < VE_MAX ? nve : VE_MAX;
           for (int z = 0; z < seen; ++z)
-            if (ve[z * 256] == h.hit) {
+            if (*ve_slot(lds, tx, z) == h.hit) {
               dup = true;
               break;
             }
           if (dup) continue;
-          if (nve < 8) ve[nve * 256] = h.hit;
+          if (nve < VE_MAX) *ve_slot(lds, tx, nve) = h.hit;
           else err |= GDEM_ERR_VE_OVERFLOW_BIT;
           ++nve;
         }
