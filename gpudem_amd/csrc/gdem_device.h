@@ -201,9 +201,11 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
 // Global layout is slot-major: keys [cap][N] (int) and deltas [cap][N]
 // (4-vector, w unused), so lane i of a wave touches consecutive words.
 // HistLds keeps the live keys in an LDS column of the thread (CAP >= the
-// scene's capacity): they are loaded once per step with independent loads,
-// and the per-contact search costs no memory round trip.  HistScan searches
-// the keys in memory and serves capacities beyond the LDS budget.
+// scene's capacity): a wave's slot row is 64 consecutive words in memory and
+// in LDS, so the keys are loaded straight into LDS (global_load_lds) without
+// occupying registers, while the screening pass runs; rows_ready() waits for
+// them.  The per-contact search then costs no memory round trip.  HistScan
+// searches the keys in memory and serves capacities beyond the LDS budget.
 template <class H, int CAP>
 struct HistLds {
   using H4 = typename Vec4<H>::type;
@@ -212,40 +214,30 @@ struct HistLds {
   int s, N, cap;
   int n;           // live slots
   uint64_t marks;  // slots in contact this step
-  int* k;          // this thread's LDS column: k[q * 256]
-  int* k0;         // thread 0's column; thread t's column is k0 + t
-  // ok: this lane's particle is active (other lanes load nothing of theirs).
-  // fetch issues the key loads into registers and keep stores them in LDS,
-  // so the loads of several rows can share one memory round trip.
-  __device__ __forceinline__ void fetch(int nh, bool ok, int (&kk)[CAP]) {
+  int* k0;         // thread 0's column; thread t's column is k0 + t, slot q at [q * 256]
+  // ok: this lane's particle is active.  The wave loads as many slots as its
+  // longest row (never more than the capacity); lanes without a particle read
+  // slot row 0 of particle 0 instead, so every address is inside the [cap][N]
+  // array.  Slots past a lane's own row hold other keys and are never
+  // searched.
+  // prepare returns the number of slot rows the wave loads; issue loads them
+  // (split so that the wave shuffles of several rows come before the first
+  // LDS load: the compiler makes any LDS access after such a load wait for it)
+  __device__ __forceinline__ int prepare(int nh, bool ok) {
     n = nh;
     marks = 0;
-    // The wave loads as many slots as its longest row (never more than the
-    // capacity); lanes without a particle read slot row 0 of particle 0
-    // instead, so every address is inside the [cap][N] array.  Slots past a
-    // lane's own row are not stored.
     int wmax = ok ? nh : 0;
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
-    wmax = min(__builtin_amdgcn_readfirstlane(wmax), min(cap, CAP));
+    return min(__builtin_amdgcn_readfirstlane(wmax), min(cap, CAP));
+  }
+  __device__ __forceinline__ void issue(int wmax, bool ok) {
     const int* src = gkey + (ok ? s : 0);
-#pragma unroll
-    for (int q = 0; q < CAP; ++q) kk[q] = q < wmax ? ld_nt(src + (size_t)q * N) : 0;
+    int* dst = k0 + (threadIdx.x & ~63);  // this wave's part of slot row 0
+    for (int q = 0; q < wmax; ++q)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)q * N),
+                                       (__attribute__((address_space(3))) void*)(dst + q * 256), 4, 0, 0);
   }
-  __device__ __forceinline__ void keep(const int (&kk)[CAP]) {
-#pragma unroll
-    for (int q = 0; q < CAP; ++q)
-      if (q < n) k[q * 256] = kk[q];
-  }
-  __device__ __forceinline__ void load(int nh, bool ok) {
-    int kk[CAP];
-    fetch(nh, ok, kk);
-    keep(kk);
-  }
-  __device__ __forceinline__ int find(int key) const {
-    for (int q = 0; q < n; ++q)
-      if (k[q * 256] == key) return q;
-    return -1;
-  }
+  __device__ __forceinline__ void load(int nh, bool ok) { issue(prepare(nh, ok), ok); }
   // slot of `key` among the first n_o slots of the particle held by thread
   // col (slot index so)
   __device__ __forceinline__ int find_in(int col, int so, int n_o, int key) const {
@@ -254,7 +246,6 @@ struct HistLds {
       if (c[q * 256] == key) return q;
     return -1;
   }
-  __device__ __forceinline__ void remember(int key) { k[n * 256] = key; }
 };
 
 template <class H>
@@ -269,18 +260,22 @@ struct HistScan {
     n = nh;
     marks = 0;
   }
-  __device__ __forceinline__ int find(int key) const {
-    for (int q = 0; q < n; ++q)
-      if (gkey[(size_t)q * N + s] == key) return q;
-    return -1;
-  }
   __device__ __forceinline__ int find_in(int, int so, int n_o, int key) const {
     for (int q = 0; q < n_o; ++q)
       if (gkey[(size_t)q * N + so] == key) return q;
     return -1;
   }
-  __device__ __forceinline__ void remember(int) {}
 };
+
+// Waits until the rows' keys are in LDS (the wave's own LDS loads have
+// landed: s_waitcnt vmcnt(0)) before any lane searches another lane's column.
+template <class Row>
+__device__ __forceinline__ void rows_ready(Row&) {}
+template <class H, int CAP>
+__device__ __forceinline__ void rows_ready(HistLds<H, CAP>&) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt, lgkmcnt not waited for
+  __builtin_amdgcn_wave_barrier();
+}
 
 // Tangential spring with Coulomb cap and history (d_calc_tangential_force,
 // device_dem.cu:622-740; the wall twin :403-499 is identical with the wall
@@ -325,47 +320,6 @@ __device__ __forceinline__ void store_delta(typename Vec4<H>::type* p, V3<T> dn)
   o.z = (decltype(o.z))dn.z;
   o.w = 0;
   st_nt(o, p);
-}
-
-// Tangential spring with Coulomb cap and history (d_calc_tangential_force,
-// device_dem.cu:622-740; the wall twin :403-499 is identical with the wall
-// history).  The slot of `key` is found or appended (d_calc_tangential_force
-// :630-653); T is the force type and FP64 keeps the reference's operations.
-template <class T, class Row>
-__device__ __forceinline__ V3<T> tangential_at(Row& row, uint32_t& err, uint32_t ovf_bit, int key, int slot,
-                                               V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu,
-                                               T dt) {
-  using H = decltype(row.gd->x);
-  if (slot < 0) {
-    // new contact: appended in list order (old is zero)
-    if (row.n < row.cap) {
-      slot = row.n;
-      st_nt(key, row.gkey + (size_t)slot * row.N + row.s);
-      row.remember(key);
-      ++row.n;
-    } else {
-      err |= ovf_bit;
-    }
-  }
-  if (slot >= 0) row.marks |= (uint64_t)1 << slot;
-  V3<T> dn;
-  const V3<T> ft = tangential_force<T>(old, fn, vt, n, eta, k, mu, dt, dn);
-  if (slot >= 0) store_delta<H, T>(row.gd + (size_t)slot * row.N + row.s, dn);
-  return ft;
-}
-
-// Slot lookup + history read, then tangential_at.
-template <class T, class Row>
-__device__ __forceinline__ V3<T> tangential(Row& row, uint32_t& err, uint32_t ovf_bit, int key, V3<T> fn,
-                                            V3<T> vt, V3<T> n, T eta, T k, T mu, T dt) {
-  using H4 = typename Row::H4;
-  const int slot = row.find(key);
-  V3<T> old{T(0), T(0), T(0)};
-  if (slot >= 0) {
-    const H4 d = ld_nt(row.gd + (size_t)slot * row.N + row.s);
-    old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
-  }
-  return tangential_at<T>(row, err, ovf_bit, key, slot, old, fn, vt, n, eta, k, mu, dt);
 }
 
 // Slots from the first one without a contact this step are dropped
@@ -450,13 +404,12 @@ __device__ __forceinline__ int* ve_slot(Lds& lds, int tx, int u) {
 template <class Row>
 __device__ __forceinline__ void bind_lds(Row&, int*, int) {}
 template <class H, int CAP>
-__device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col0, int tx) {
+__device__ __forceinline__ void bind_lds(HistLds<H, CAP>& r, int* col0, int) {
   r.k0 = col0;
-  r.k = col0 + tx;
 }
 
-// Loads the pair and wall history rows of a particle; LDS rows issue all
-// their key loads before the first LDS store.
+// Starts loading the pair and wall history rows of a particle (rows_ready()
+// before the keys are searched).
 template <class A, class B>
 __device__ __forceinline__ void load_rows(A& a, int na, B& b, int nb, bool ok) {
   a.load(na, ok);
@@ -464,11 +417,9 @@ __device__ __forceinline__ void load_rows(A& a, int na, B& b, int nb, bool ok) {
 }
 template <class H, int CA, int CB>
 __device__ __forceinline__ void load_rows(HistLds<H, CA>& a, int na, HistLds<H, CB>& b, int nb, bool ok) {
-  int ka[CA], kb[CB];
-  a.fetch(na, ok, ka);
-  b.fetch(nb, ok, kb);
-  a.keep(ka);
-  b.keep(kb);
+  const int wa = a.prepare(na, ok), wb = b.prepare(nb, ok);
+  a.issue(wa, ok);
+  b.issue(wb, ok);
 }
 
 template <class Row>
@@ -669,6 +620,18 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   int nrec = 0;
   const int self_id = (int)w.w;
 
+  // The wave's wall candidates are spread over its lanes, 64 at a time; the
+  // first 64 are fetched now so the load overlaps the pair pass.
+  const int w_incl = wave_incl_sum(nwl);
+  const int w_total = __builtin_amdgcn_readfirstlane(__shfl(w_incl, 63));
+  int w_t0 = 0;
+  {
+    // shuffles run with every lane active
+    const int o = owner_lane(w_incl, lane);
+    const int o_first = __shfl(w_incl - nwl, o), so = __shfl(s, o);
+    if (lane < w_total) w_t0 = ld_nt(A.wl + (size_t)(lane - o_first) * N + so);
+  }
+
   // history rows of pairs and walls
   PRow prow;
   bind_lds(prow, &lds.pkey[0][0], tx);
@@ -684,21 +647,10 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   wrow.s = s;
   wrow.N = N;
   wrow.cap = A.whist_cap;
+  // (after the wave shuffles above: an LDS access after the loads below waits
+  // for them, and the first one is the screening pass's, which waits for its
+  // own candidate loads anyway)
   load_rows(prow, nh0, wrow, nw0, act);
-  // the keys are read from other lanes' columns below
-  __builtin_amdgcn_wave_barrier();
-  // The wave's wall candidates are spread over its lanes, 64 at a time; the
-  // first 64 are fetched now so the load overlaps the pair pass.
-  const int w_incl = wave_incl_sum(nwl);
-  const int w_total = __builtin_amdgcn_readfirstlane(__shfl(w_incl, 63));
-  int w_t0 = 0;
-  {
-    // shuffles run with every lane active
-    const int o = owner_lane(w_incl, lane);
-    const int o_first = __shfl(w_incl - nwl, o), so = __shfl(s, o);
-    if (lane < w_total) w_t0 = ld_nt(A.wl + (size_t)(lane - o_first) * N + so);
-  }
-
   // ---- particle pairs -------------------------------------------------------
   {
     PRow& row = prow;
@@ -730,6 +682,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     }
     // the entries are read from other lanes' columns below
     __builtin_amdgcn_wave_barrier();
+    // the keys are searched in other lanes' columns from here on
+    rows_ready(prow);
+    rows_ready(wrow);
     // the touching pairs, spread over the lanes
     const int cnt = __popcll(hits);
     const int incl = wave_incl_sum(cnt);
@@ -864,15 +819,24 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   }
 
   // ---- walls ----------------------------------------------------------------
+  // The wave's wall candidates are spread over the lanes, 64 at a time, like
+  // the touching pairs: each lane tests its candidate's distance and, for a
+  // contact, evaluates the contact force.  A vertex or edge contact whose
+  // feature an earlier contact of the same particle (in list order) already
+  // touched is skipped (d_particle_collision_verlet's hitAt bookkeeping);
+  // new history slots go to the kept contacts in list order; each particle
+  // then adds its kept contacts in list order.
   {
     WRow& row = wrow;
     // (every lane's pair pass is done before the wall pass writes the union)
     __builtin_amdgcn_wave_barrier();
-    // vertex/edge features touched this step: *ve_slot(lds, tx, u), u < VE_MAX
+    // features of this particle's kept vertex/edge contacts of earlier rounds:
+    // *ve_slot(lds, tx, u), u < min(nve, VE_MAX)
     int nve = 0;
+    int added = 0;  // new history slots this particle's contacts have taken
+    uint64_t marks = 0;
     const double rsq = ri * ri;
     const T eta = etac * sqrtm;
-    // distance tests of the wave's wall candidates, spread over the lanes
     const int incl = w_incl, total = w_total, first = incl - nwl;
     for (int base = 0; base < total; base += 64) {
       const int g = base + lane;
@@ -881,87 +845,125 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o_first = __shfl(first, o);
       const int so = __shfl(s, o);
       const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
-      const double orsq = __shfl(rsq, o);
-      const int o_n = __shfl(nw0, o);
-      int t = 0, slot = -1;
-      H4 d{};
-      TriHit th;
-      th.n = V3<double>{0., 0., 0.};
-      th.dist = 1.e10;
-      th.hit = -1;
+      const double orsq = __shfl(rsq, o), ori = __shfl(ri, o);
+      const int o_n = __shfl(nw0, o), o_added = __shfl(added, o), o_nve = __shfl(nve, o);
+      int t = 0, slot = -1, hit = -1;
+      double dist = 1.e10;
+      V3<T> nT{T(0), T(0), T(0)}, old{T(0), T(0), T(0)};
       if (has) {
         // candidate, its history slot and displacement, and its distance
         t = base == 0 ? w_t0 : ld_nt(A.wl + (size_t)(g - o_first) * N + so);
         slot = row.find_in(wtx + o, so, o_n, t);
-        if (slot >= 0) d = ld_nt(row.gd + (size_t)slot * N + so);
-        th = tri_distance(A.tri + t, opx, opy, opz, orsq);
+        if (slot >= 0) {
+          const H4 d = ld_nt(row.gd + (size_t)slot * N + so);
+          old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
+        }
+        const TriHit th = tri_distance(A.tri + t, opx, opy, opz, orsq);
+        dist = th.dist;
+        hit = th.hit;
+        nT = V3<T>{(T)th.n.x, (T)th.n.y, (T)th.n.z};
       }
-      // every particle handles its candidates of this round in list order
+      const bool con = has && dist < ori;
+      // this particle's candidates of the round are lanes [my_lo, my_hi);
+      // the candidate's owner's are [c_lo, ...)
       const int my_lo = max(first - base, 0), my_hi = min(first + nwl - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
-      const int rounds = wave_max(my_n);
-      for (int u = 0; u < rounds; ++u) {
+      const int span = wave_max(my_n);
+      const int c_lo = max(o_first - base, 0);
+      bool dup = false;
+      if (con && hit != -1) {
+        for (int z = 0; z < min(o_nve, VE_MAX); ++z)
+          if (*ve_slot(lds, wtx + o, z) == hit) dup = true;
+      }
+      for (int kk = 1; kk < span; ++kk) {
+        const int src = max(lane - kk, 0);
+        const int c_src = __shfl((int)con, src), h_src = __shfl(hit, src);
+        if (lane - kk >= c_lo && con && hit != -1 && c_src && h_src == hit) dup = true;
+      }
+      const bool keep = con && !dup;
+      // new contacts take their particle's next free slots in list order
+      const uint64_t fresh = __ballot(keep && slot < 0);
+      int wslot = slot;
+      if (keep && slot < 0) {
+        const int ns = o_n + o_added + __popcll(fresh & lane_range(c_lo, lane));
+        if (ns < row.cap) {
+          wslot = ns;
+          st_nt(t, row.gkey + (size_t)ns * N + so);
+        } else {
+          err |= GDEM_ERR_WHIST_OVERFLOW_BIT;
+        }
+      }
+      const T ok = __shfl(k, o), oeta = __shfl(eta, o), oriT = __shfl(riT, o);
+      const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
+      const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
+      V3<T> fn{T(0), T(0), T(0)}, ft{T(0), T(0), T(0)}, mm{T(0), T(0), T(0)};
+      double delMagD = 0.;
+      if (keep) {
+        delMagD = ori - dist;
+        const T delMag = (T)delMagD;
+        const V3<T> n = nT;
+        const V3<T> del = scal(-delMag, n);
+        const T vrn = dot(ovi, n);
+        const V3<T> vn = scal(vrn, n);
+        fn.x = -ok * del.x - oeta * vn.x;
+        fn.y = -ok * del.y - oeta * vn.y;
+        fn.z = -ok * del.z - oeta * vn.z;
+        V3<T> vrot{oriT * owi.x, oriT * owi.y, oriT * owi.z};
+        vrot = cross(vrot, n);
+        const V3<T> vt = vadd(vsub(ovi, vn), vrot);
+        V3<T> dn;
+        ft = tangential_force<T>(old, fn, vt, n, oeta, ok, mu, dtT, dn);
+        if (wslot >= 0) store_delta<H, T>(row.gd + (size_t)wslot * N + so, dn);
+        mm = scal(oriT, cross(n, ft));
+      }
+      // every particle adds its kept contacts of this round in list order
+      for (int u = 0; u < span; ++u) {
         const bool mine = u < my_n;
         const int src = mine ? my_lo + u : lane;
-        TriHit h;
-        h.n = V3<double>{__shfl(th.n.x, src), __shfl(th.n.y, src), __shfl(th.n.z, src)};
-        h.dist = __shfl(th.dist, src);
-        h.hit = __shfl(th.hit, src);
-        const int tt = __shfl(t, src);
-        const int hs = __shfl(slot, src);
-        const V3<T> old{__shfl((T)d.x, src), __shfl((T)d.y, src), __shfl((T)d.z, src)};
-        if (!mine || !(h.dist < ri)) continue;
-        if (h.hit != -1) {
-          bool dup = false;
-          const int seen = nve < VE_MAX ? nve : VE_MAX;
-          for (int z = 0; z < seen; ++z)
-            if (*ve_slot(lds, tx, z) == h.hit) {
-              dup = true;
-              break;
-            }
-          if (dup) continue;
-          if (nve < VE_MAX) *ve_slot(lds, tx, nve) = h.hit;
-          else err |= GDEM_ERR_VE_OVERFLOW_BIT;
-          ++nve;
-        }
-        const double delMagD = ri - h.dist;
-        const T delMag = (T)delMagD;
-        const V3<T> n{(T)h.n.x, (T)h.n.y, (T)h.n.z};
-        const V3<T> del = scal(-delMag, n);
-        const T vrn = dot(vi, n);
-        const V3<T> vn = scal(vrn, n);
-        V3<T> fn;
-        fn.x = -k * del.x - eta * vn.x;
-        fn.y = -k * del.y - eta * vn.y;
-        fn.z = -k * del.z - eta * vn.z;
-        V3<T> vrot{riT * wi.x, riT * wi.y, riT * wi.z};
-        vrot = cross(vrot, n);
-        const V3<T> vt = vadd(vsub(vi, vn), vrot);
-        f.x += fn.x;
-        f.y += fn.y;
-        f.z += fn.z;
-        const V3<T> ft = tangential_at<T>(row, err, GDEM_ERR_WHIST_OVERFLOW_BIT, tt, hs,
-                                          hs >= 0 ? old : V3<T>{T(0), T(0), T(0)}, fn, vt, n, eta, k, mu, dtT);
-        f.x += ft.x;
-        f.y += ft.y;
-        f.z += ft.z;
-        const V3<T> mm = scal(riT, cross(n, ft));
-        mom.x += mm.x;
-        mom.y += mm.y;
-        mom.z += mm.z;
+        const int kp = __shfl((int)keep, src);
+        const T a0 = __shfl(fn.x, src), a1 = __shfl(fn.y, src), a2 = __shfl(fn.z, src);
+        const T b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        const T c0 = __shfl(mm.x, src), c1 = __shfl(mm.y, src), c2 = __shfl(mm.z, src);
+        const int ws = __shfl(wslot, src), hh = __shfl(hit, src);
+        int rkey = 0;
+        double rdel = 0.;
         if constexpr (REC) {
-          if (nrec < rec.cap) {
-            const size_t r = (size_t)self_id * rec.cap + nrec;
-            rec.key[r] = tt;
-            rec.kind[r] = 1;
-            rec.fn[r * 3] = fn.x; rec.fn[r * 3 + 1] = fn.y; rec.fn[r * 3 + 2] = fn.z;
-            rec.ft[r * 3] = ft.x; rec.ft[r * 3 + 1] = ft.y; rec.ft[r * 3 + 2] = ft.z;
-            rec.delta[r] = delMagD;
-            ++nrec;
+          rkey = __shfl(t, src);
+          rdel = __shfl(delMagD, src);
+        }
+        if (mine && kp) {
+          f.x += a0;
+          f.y += a1;
+          f.z += a2;
+          f.x += b0;
+          f.y += b1;
+          f.z += b2;
+          mom.x += c0;
+          mom.y += c1;
+          mom.z += c2;
+          if (ws >= 0) marks |= (uint64_t)1 << ws;
+          if (hh != -1) {
+            if (nve < VE_MAX) *ve_slot(lds, tx, nve) = hh;
+            else err |= GDEM_ERR_VE_OVERFLOW_BIT;
+            ++nve;
+          }
+          if constexpr (REC) {
+            if (nrec < rec.cap) {
+              const size_t r = (size_t)self_id * rec.cap + nrec;
+              rec.key[r] = rkey;
+              rec.kind[r] = 1;
+              rec.fn[r * 3] = a0; rec.fn[r * 3 + 1] = a1; rec.fn[r * 3 + 2] = a2;
+              rec.ft[r * 3] = b0; rec.ft[r * 3 + 1] = b1; rec.ft[r * 3 + 2] = b2;
+              rec.delta[r] = rdel;
+              ++nrec;
+            }
           }
         }
       }
+      added += __popcll(fresh & lane_range(my_lo, my_hi));
     }
+    row.n = min(nw0 + added, row.cap);
+    row.marks = marks;
     if (act) st_nt(truncate_history(row.n, row.marks), A.nw + s);
   }
   if constexpr (REC) {
