@@ -194,8 +194,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--graph-steps", type=int, default=0,
-                    help="> 0: launch-chain rebuild captured in graphs of this many steps; "
-                         "< 0: launch-chain rebuild, direct launches")
+                    help="> 0: steps captured per hipGraph; 0: direct launches; "
+                         "< 0: rebuild as per-phase kernels, direct launches")
     ap.add_argument("--skin-factor", type=float, default=None,
                     help="chute Verlet skin in units of r_max (refresh = skin/2, cell = 2 r_max + skin)")
     ap.add_argument("--wall-broadphase", type=int, default=0,
@@ -267,7 +267,7 @@ def main():
         "config": {"workload": "helical_chute_1M_20k_tris_mixed", "particles": spec.n,
                    "triangles": int(len(spec.verts)), "dt": spec.params["dt"],
                    "skin": spec.params["skin"], "cell": spec.params["cell"],
-                   "rebuild": "cooperative" if args.graph_steps == 0 else "chain",
+                   "rebuild": "per-phase kernels" if args.graph_steps < 0 else "grid-barrier kernel",
                    "wall_broadphase": ["table", "bvh", "table"][args.wall_broadphase]},
         "list_stats": {"mean_nb": s1["mean_nb"], "mean_wall": s1["mean_wall"],
                        "mean_contacts": s1["mean_contacts"], "rebuilds_in_window": m["rebuilds"],
