@@ -464,12 +464,11 @@ struct StepArgs {
   double threshSq;
 };
 
-// A particle's own records: state, material, reference position, list and
-// history lengths.  All loads are independent, so they share one memory round
+// A particle's own records: state, material, list and history lengths.  All loads are independent, so they share one memory round
 // trip; lanes without a particle load nothing.
 template <class H>
 struct Own {
-  double4 p, v, w, rf;
+  double4 p, v, w, rf;  // rf (reference position) only in the mixed form
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
 };
@@ -488,7 +487,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     o.v = A.vin[s];
     o.w = A.win[s];
     o.mt = ld_nt(A.mat + s);
-    o.rf = ld_nt(A.ref + s);
+    if constexpr (sizeof(H) != 8) o.rf = ld_nt(A.ref + s);
     o.nnb = ld_nt(A.nnb + s);
     o.nwl = ld_nt(A.nwl + s);
     o.nh0 = ld_nt(A.nh + s);
@@ -606,7 +605,6 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     st_nt(w, A.wout + s);
   }
   const H4 mt = own.mt;
-  const double4 rf = own.rf;  // position at the last list build
   const int nnb = act ? own.nnb : 0, nwl = act ? own.nwl : 0;
   const int nh0 = act ? own.nh0 : 0, nw0 = act ? own.nw0 : 0;
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
@@ -701,8 +699,10 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int so = __shfl(s, o);
       const int o_n = __shfl(nh0, o);
       const int o_added = __shfl(added, o);
-      const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
-      const double ori = __shfl(ri, o), oinvm = __shfl(invm, o);
+      // the owner's position record, from the workgroup's tile
+      const double4 op = ld_nt(tile_p + wtx + o);
+      const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
+      const double oinvm = __shfl(invm, o);
       const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
       const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
       const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
@@ -835,7 +835,6 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     int nve = 0;
     int added = 0;  // new history slots this particle's contacts have taken
     uint64_t marks = 0;
-    const double rsq = ri * ri;
     const T eta = etac * sqrtm;
     const int incl = w_incl, total = w_total, first = incl - nwl;
     for (int base = 0; base < total; base += 64) {
@@ -844,8 +843,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
       const int so = __shfl(s, o);
-      const double opx = __shfl(p.x, o), opy = __shfl(p.y, o), opz = __shfl(p.z, o);
-      const double orsq = __shfl(rsq, o), ori = __shfl(ri, o);
+      const double4 op = ld_nt(tile_p + wtx + o);  // the owner's position record
+      const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
+      const double orsq = ori * ori;
       const int o_n = __shfl(nw0, o), o_added = __shfl(added, o), o_nve = __shfl(nve, o);
       int t = 0, slot = -1, hit = -1;
       double dist = 1.e10;
@@ -972,18 +972,37 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   if (!act) return;
 
   // ---- integration (k_integrate), out-of-bounds, refresh check --------------
+  // The FP64 form reads the particle's own records again here (the position
+  // from the tile, the rest from memory, where they are still cached) rather
+  // than holding them in registers through the contact passes, which would
+  // spill at its occupancy bound; the empty asm keeps the compiler from
+  // reusing the values loaded at the start.  The mixed form keeps them (one
+  // round trip fewer, measured faster).
+  double4 p1, v1, w1, rf;  // rf: position at the last list build
+  if constexpr (sizeof(T) == 8) {
+    asm volatile("" ::: "memory");
+    p1 = ld_nt(tile_p + tx);
+    v1 = vin[s];
+    w1 = win[s];
+    rf = ld_nt(A.ref + s);
+  } else {
+    p1 = p;
+    v1 = v;
+    w1 = w;
+    rf = own.rf;
+  }
   const double dt = A.dt;
   const double ax = (double)f.x * invm + A.g[0];
   const double ay = (double)f.y * invm + A.g[1];
   const double az = (double)f.z * invm + A.g[2];
   double4 vo, po, wo;
-  vo.x = v.x + ax * dt;
-  vo.y = v.y + ay * dt;
-  vo.z = v.z + az * dt;
-  vo.w = v.w;
-  po.x = p.x + vo.x * dt;
-  po.y = p.y + vo.y * dt;
-  po.z = p.z + vo.z * dt;
+  vo.x = v1.x + ax * dt;
+  vo.y = v1.y + ay * dt;
+  vo.z = v1.z + az * dt;
+  vo.w = v1.w;
+  po.x = p1.x + vo.x * dt;
+  po.y = p1.y + vo.y * dt;
+  po.z = p1.z + vo.z * dt;
   const double bx = (double)mom.x * invmoi;
   const double by = (double)mom.y * invmoi;
   const double bz = (double)mom.z * invmoi;
@@ -999,13 +1018,13 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       rec.anga[(size_t)self_id * 3 + 2] = bz;
     }
   }
-  wo.x = w.x + bx * dt;
-  wo.y = w.y + by * dt;
-  wo.z = w.z + bz * dt;
-  wo.w = w.w;
+  wo.x = w1.x + bx * dt;
+  wo.y = w1.y + by * dt;
+  wo.z = w1.z + bz * dt;
+  wo.w = w1.w;
   const bool out = po.x < A.bmin[0] || po.x > A.bmax[0] || po.y < A.bmin[1] || po.y > A.bmax[1] ||
                    po.z < A.bmin[2] || po.z > A.bmax[2];
-  po.w = out ? -ri : ri;
+  po.w = out ? -p1.w : p1.w;
   if (!out) {
     const double ex = po.x - rf.x, ey = po.y - rf.y, ez = po.z - rf.z;
     refresh = (ex * ex + ey * ey + ez * ez) > A.threshSq;
