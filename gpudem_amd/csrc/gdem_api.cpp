@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -199,6 +200,10 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
                          : std::max(1, std::min(1024, (N + 2047) / 2048));
   P.rs_chunk = ((N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
   P.list_blocks = list_grid<H>(N);
+  // GDEM_GBAR_SPIN (environment) lowers the barrier's poll limit, to exercise
+  // its timeout path; unset, the default holds.
+  P.gbar_spin = GBAR_SPIN;
+  if (const char* e = std::getenv("GDEM_GBAR_SPIN")) P.gbar_spin = std::max(0, std::atoi(e));
   int rc = alloc_bufs<H>(sc, B, ncell);
   if (rc) return rc;
   using H4 = typename Vec4<H>::type;

@@ -23,9 +23,12 @@ constexpr int RS_DIGITS = 1 << RS_BITS;
 constexpr int RS_PASSES = 3;
 constexpr int RS_FINAL = RS_PASSES & 1;
 // Two-level grid barrier of the rebuild kernel: BAR_SUB arrival
-// counters, a top counter and a generation word, each on its own 128 B line.
+// counters, a top counter, a generation word and an abort word, each on its
+// own 128 B line.
 constexpr int BAR_SUB = 32;
-constexpr int GBAR_WORDS = (BAR_SUB + 2) * 32;
+constexpr int GBAR_ABORT = (BAR_SUB + 2) * 32;
+constexpr int GBAR_WORDS = (BAR_SUB + 3) * 32;
+constexpr int GBAR_SPIN = 1 << 20;  // default poll limit of a barrier wait (about a second)
 
 struct RecDev {
   int cap;

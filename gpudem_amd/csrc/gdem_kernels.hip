@@ -262,8 +262,10 @@ __device__ __forceinline__ void rs_scatter_block(int n, int chunk, int pass, con
     __syncthreads();  // every base read before the updates below
     if (valid) {
       const uint32_t pos = bd + before + (uint32_t)rank;
-      kout[pos] = k;
-      iout[pos] = idx;
+      if (pos < (uint32_t)n) {  // always, unless a grid barrier failed
+        kout[pos] = k;
+        iout[pos] = idx;
+      }
       if (rank == 0 && last_wave) base[d] = bd + before + cw;
     }
     __syncthreads();
@@ -276,6 +278,7 @@ template <class H>
 __device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B, int s, int cur, int hc) {
   const int N = P.N;
   const int src = B.idx[RS_FINAL][s];
+  if ((unsigned)src >= (unsigned)N) return;  // always in range, unless a grid barrier failed
   B.pos[cur ^ 1][s] = B.pos[cur][src];
   B.vel[cur ^ 1][s] = B.vel[cur][src];
   B.angv[cur ^ 1][s] = B.angv[cur][src];
@@ -441,12 +444,15 @@ __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
 // arrivals (about 0.1 us each on MI355X).  Each workgroup releases its writes
 // before arriving and acquires after the generation moves; the counters are
 // reset before the generation word is released, so the next barrier finds
-// them at zero.  The wait is bounded: a barrier that does not complete within
-// about a second sets GDEM_ERR_BARRIER and lets the workgroup go on (the
-// rebuild's results are then invalid and the error is reported).
-constexpr int GBAR_SPIN = 1 << 20;
-
-__device__ __forceinline__ void grid_barrier(uint32_t* gbar, int* ctl) {
+// them at zero.  The wait is bounded (spin polls, about a second by default):
+// a workgroup whose barrier does not complete sets GDEM_ERR_BARRIER and the
+// sticky abort word, and returns false, and so does every workgroup that
+// finds the abort word set after a barrier; the kernel then ends without
+// using the incomplete phase's data (its scatter and permutation also reject
+// out-of-range positions), and every later rebuild of the scene aborts at
+// once.  The error is reported by gdem_scene_status.
+__device__ __forceinline__ bool grid_barrier(uint32_t* gbar, int* ctl, int spin) {
+  __shared__ int go;
   __syncthreads();
   if (threadIdx.x == 0) {
     const int G = gridDim.x, b = blockIdx.x;
@@ -468,16 +474,23 @@ __device__ __forceinline__ void grid_barrier(uint32_t* gbar, int* ctl) {
         __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
+    uint32_t* abort_word = gbar + GBAR_ABORT;
+    int ok = 1;
     for (int it = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0; ++it) {
-      if (it > GBAR_SPIN) {
+      if (it >= spin) {
         atomicOr((unsigned int*)&ctl[CTL_ERR], GDEM_ERR_BARRIER_BIT);
+        __hip_atomic_store(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
         break;
       }
       __builtin_amdgcn_s_sleep(2);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
+    go = ok;
   }
   __syncthreads();
+  return go != 0;
 }
 
 // Keys, the three radix passes, the permutation and the cell table need grid
@@ -489,6 +502,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* gbar, int* ctl) {
 template <class H>
 __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
+  if (__builtin_amdgcn_readfirstlane(B.gbar[GBAR_ABORT]) != 0) return;  // an earlier barrier timed out
   const int cur = B.ctl[CTL_CUR];
   const int hc = B.ctl[CTL_HCUR];
   const int N = P.N;
@@ -508,18 +522,18 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
     for (int s = beg + threadIdx.x; s < end; s += RS_THREADS) atomicAdd(&dig[rs_digit(keys_one(P, B, s, cur), 0)], 1u);
   }
   rs_store_counts(dig, hist, b, G);
-  grid_barrier(B.gbar, B.ctl);
+  if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
   for (int pass = 0; pass < RS_PASSES; ++pass) {
     const int a = pass & 1;
     if (pass > 0) {
       rs_count_block(N, P.rs_chunk, pass, B.key[a], hist, b, G, dig);
-      grid_barrier(B.gbar, B.ctl);
+      if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
     }
     rs_scan_digits(hist, dtot, G, b, G);
-    grid_barrier(B.gbar, B.ctl);
+    if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
     rs_bases(hist, dtot, b, G, dig);
     rs_scatter_block(N, P.rs_chunk, pass, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1], dig, wcnt, b);
-    grid_barrier(B.gbar, B.ctl);
+    if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
   }
   // The cell table needs the sorted keys (complete at the last barrier) and
   // the cell of each slot, which this thread's own permute step wrote; so

@@ -47,6 +47,7 @@ struct SimParams {
   int ntri, nnode;
   int rs_blocks, rs_chunk;  // radix sort grid and per-block chunk
   int list_blocks;          // grid of the pair-list kernel (co-resident workgroups, grid-stride)
+  int gbar_spin;            // poll limit of the rebuild kernel's grid barriers
   int wall_table;           // 1: wall candidates from the static per-cell table, 0: BVH traversal
 };
 
