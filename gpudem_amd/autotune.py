@@ -5,8 +5,10 @@ cell >= 2 r_max + skin, so the Verlet skin, the cell size and the number of
 steps per captured hipGraph change only the speed, never a trajectory (the
 GPU tests check this bit for bit).  The tuner times each candidate on a
 sample of the real workload -- a few warm-up steps, then a timed window --
-and keeps the fastest.  The workgroup size is fixed at 256 (the step kernel's
-LDS history columns and the XCD tile mapping are laid out for it).
+and keeps the fastest.  The window starts after 1,000 steps by default: the
+first few hundred steps of freshly admitted scenes (no contacts yet) favour
+settings that lose later.  The workgroup size is fixed at 256 (the step
+kernel's LDS history columns and the XCD tile mapping are laid out for it).
 """
 import itertools
 import time
@@ -52,7 +54,7 @@ def time_batch(ctx, specs, precision, skin_factor, cell_factor, graph_steps, war
     return ms / steps, total * steps / (ms / 1e3)
 
 
-def tune_batch(ctx, specs, precision, sample=16, warm=100, steps=300, caps=None, candidates=None, log=None):
+def tune_batch(ctx, specs, precision, sample=32, warm=1000, steps=1000, caps=None, candidates=None, log=None):
     """Times every (skin factor, cell factor, graph steps) candidate on every
     k-th scene of `specs` (about `sample` scenes).  Returns (best, table) where
     best = dict(skin_factor, cell_factor, graph_steps) and table lists
