@@ -195,6 +195,99 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
   return tri_distance_g(*tg, px, py, pz, Rsq);
 }
 
+// The same test in two memory round trips, for the contact pass.  Reading the
+// fields as tri_distance_g needs them costs up to four dependent trips
+// (plane; barycentric terms; region data; feature id).  Here the first trip
+// requests the plane, the first vertex, both edges from it and the
+// barycentric terms together (an empty asm uses them at once, so the
+// compiler cannot sink the later ones into the branches after the plane
+// test); a point outside the face region then requests its region's data in
+// one second trip: the base point, edge, inverse squared length and feature
+// id, at offsets selected per lane.  The arithmetic is tri_distance_g's,
+// expression for expression: the edge parameter dot(pos - P, E) * inv equals
+// its d20 * d00inv and d21 * d11inv for the edges from the first vertex.
+__device__ __forceinline__ TriHit tri_distance_2trip(const TriGeo* __restrict__ g, double px, double py,
+                                                     double pz, double Rsq) {
+  TriHit res;
+  res.n = V3<double>{0., 0., 0.};
+  res.dist = 1.e10;
+  res.hit = -1;
+  const V3<double> nt{g->n[0], g->n[1], g->n[2]};
+  const double pd = g->d;
+  const V3<double> P0{g->p0[0], g->p0[1], g->p0[2]};
+  const V3<double> E01{g->e01[0], g->e01[1], g->e01[2]};
+  const V3<double> E02{g->e02[0], g->e02[1], g->e02[2]};
+  const double d00 = g->d00, d01 = g->d01, d11 = g->d11, den = g->denom;
+  asm volatile("" ::"v"(nt.x), "v"(nt.y), "v"(nt.z), "v"(pd), "v"(P0.x), "v"(P0.y), "v"(P0.z));
+  asm volatile("" ::"v"(E01.x), "v"(E01.y), "v"(E01.z), "v"(E02.x), "v"(E02.y), "v"(E02.z));
+  asm volatile("" ::"v"(d00), "v"(d01), "v"(d11), "v"(den));
+  double dist = 0.;
+  dist += nt.x * px;
+  dist += nt.y * py;
+  dist += nt.z * pz;
+  dist += pd;
+  double distsq = dist * dist;
+  if (distsq > Rsq) return res;
+  const V3<double> pos{px, py, pz};
+  const V3<double> ap = vsub(pos, P0);
+  const double d20 = ap.x * E01.x + ap.y * E01.y + ap.z * E01.z;
+  const double d21 = ap.x * E02.x + ap.y * E02.y + ap.z * E02.z;
+  const double v = (d11 * d20 - d01 * d21) * den;
+  const double w = (d00 * d21 - d01 * d20) * den;
+  const double u = 1.0 - v - w;
+  if (u > 0. && v > 0. && w > 0.) {
+    const double sg = dist < 0 ? -1.0 : 1.0;
+    res.n = scal(sg, nt);
+    res.dist = sg * dist;
+    res.hit = -1;
+    return res;
+  }
+  // region: base point, edge, inverse squared edge length, feature id
+  int op, oe, oi, oh;
+  bool edge = true;
+  if (v <= 0. && w <= 0.0) {
+    op = offsetof(TriGeo, p0), oh = offsetof(TriGeo, v0), edge = false;
+  } else if (u <= 0. && w <= 0.0) {
+    op = offsetof(TriGeo, p1), oh = offsetof(TriGeo, v1), edge = false;
+  } else if (u <= 0. && v <= 0.0) {
+    op = offsetof(TriGeo, p2), oh = offsetof(TriGeo, v2), edge = false;
+  } else if (u <= 0.0) {
+    op = offsetof(TriGeo, p1), oh = offsetof(TriGeo, e1);
+  } else if (v <= 0.0) {
+    op = offsetof(TriGeo, p0), oh = offsetof(TriGeo, e2);
+  } else {
+    op = offsetof(TriGeo, p0), oh = offsetof(TriGeo, e0);
+  }
+  if (u <= 0.0) {
+    oe = offsetof(TriGeo, e12), oi = offsetof(TriGeo, d22inv);
+  } else if (v <= 0.0) {
+    oe = offsetof(TriGeo, e02), oi = offsetof(TriGeo, d11inv);
+  } else {
+    oe = offsetof(TriGeo, e01), oi = offsetof(TriGeo, d00inv);
+  }
+  const char* b = reinterpret_cast<const char*>(g);
+  const double* Pp = reinterpret_cast<const double*>(b + op);
+  const double* Ep = reinterpret_cast<const double*>(b + oe);
+  const V3<double> P{Pp[0], Pp[1], Pp[2]};
+  const V3<double> E{Ep[0], Ep[1], Ep[2]};
+  const double inv = *reinterpret_cast<const double*>(b + oi);
+  const int hit = *reinterpret_cast<const int*>(b + oh);
+  // (evaluated for a vertex too, and discarded, so that every load of the
+  // trip is issued before the first wait)
+  double t = dot(vsub(pos, P), E) * inv;
+  t = fmax(0., fmin(1.0, t));
+  const V3<double> qe = vadd(P, scal(t, E));
+  const V3<double> q = edge ? qe : P;
+  res.n = vsub(pos, q);
+  distsq = dot(res.n, res.n);
+  if (distsq < Rsq) {
+    res.dist = sqrt(distsq);
+    res.n = scal(1. / res.dist, res.n);
+  }
+  res.hit = hit;
+  return res;
+}
+
 // ---------------------------------------------------------------------------
 // contact-history row of one particle
 // ---------------------------------------------------------------------------
@@ -837,7 +930,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     uint64_t marks = 0;
     const T eta = etac * sqrtm;
     const int incl = w_incl, total = w_total, first = incl - nwl;
-    for (int base = 0; base < total; base += 64) {
+    // One round: candidates base + lane; t_in is this lane's candidate.
+    auto wall_round = [&](const int base, const int t_in) {
       const int g = base + lane;
       const bool has = g < total;
       const int o = owner_lane(incl, g);
@@ -852,13 +946,14 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       V3<T> nT{T(0), T(0), T(0)}, old{T(0), T(0), T(0)};
       if (has) {
         // candidate, its history slot and displacement, and its distance
-        t = base == 0 ? w_t0 : ld_nt(A.wl + (size_t)(g - o_first) * N + so);
+        t = t_in;
         slot = row.find_in(wtx + o, so, o_n, t);
-        if (slot >= 0) {
-          const H4 d = ld_nt(row.gd + (size_t)slot * N + so);
-          old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
-        }
-        const TriHit th = tri_distance(A.tri + t, opx, opy, opz, orsq);
+        // The history displacement (slot row 0, always allocated, stands in
+        // when there is none, so the load needs no branch) is requested with
+        // the triangle's first fields.
+        const H4 d = ld_nt(row.gd + (size_t)(slot >= 0 ? slot : 0) * N + so);
+        const TriHit th = tri_distance_2trip(A.tri + t, opx, opy, opz, orsq);
+        if (slot >= 0) old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
         dist = th.dist;
         hit = th.hit;
         nT = V3<T>{(T)th.n.x, (T)th.n.y, (T)th.n.z};
@@ -961,6 +1056,19 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         }
       }
       added += __popcll(fresh & lane_range(my_lo, my_hi));
+    };
+    // The first round is separate so that its candidates are the ones
+    // fetched at the start: a candidate loaded in the round itself makes the
+    // compiler wait for every earlier memory access, the pair pass's history
+    // stores included, before the key search.  Waves with more than 64 wall
+    // candidates fetch the later ones round by round.
+    if (total > 0) wall_round(0, w_t0);
+    for (int base = 64; base < total; base += 64) {
+      const int g = base + lane;
+      const int o = owner_lane(incl, g);
+      const int o_first = __shfl(first, o), so = __shfl(s, o);
+      const int tn = g < total ? ld_nt(A.wl + (size_t)(g - o_first) * N + so) : 0;
+      wall_round(base, tn);
     }
     row.n = min(nw0 + added, row.cap);
     row.marks = marks;
