@@ -783,13 +783,53 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     const int first = incl - cnt;
     int added = 0;  // new history slots this particle's pairs have asked for
     uint64_t marks = 0;
-    for (int base = 0; base < total; base += 64) {
+    // Pairs are handled 64 per round, one per lane.  Per pair: its list
+    // entry and history slot (LDS), then the partner's position, velocity
+    // and spin and the history displacement (memory).  The memory reads are
+    // issued by every lane at valid addresses (a lane without a pair reads
+    // its owner's records and slot row 0) and the arithmetic runs on every
+    // lane, so no read sits in a branch: after such a branch the compiler
+    // cannot count the reads in flight and waits for all of them, the
+    // previous round's history stores included.
+    struct PairIn {
+      int2 e;    // (partner slot, partner id)
+      int slot;  // history slot, -1 for a new contact
+      int so;    // owner's slot
+      double4 pj, vj, wj;
+      H4 d;
+    };
+    auto pair_key = [&](const int base, PairIn& q) {
       const int g = base + lane;
       const bool has = g < total;
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
       const uint64_t o_hits = shfl64(hits, o);
       const int so = __shfl(s, o);
+      const int o_n = __shfl(nh0, o);
+      const int r = g - o_first;  // rank among the owner's touching pairs
+      // (the LDS read is marked non-temporal so it stays an LDS instruction)
+      int2 e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
+      if (has && r >= HIT_STASH) {
+        e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
+        // wait for it here, inside the rare branch (see above)
+        asm volatile("" ::"v"(e.x), "v"(e.y));
+      }
+      q.so = so;
+      q.e = has ? e : make_int2(so, -1);
+      q.slot = has ? row.find_in(wtx + o, so, o_n, e.y) : -1;
+    };
+    auto pair_data = [&](PairIn& q) {
+      q.pj = pin[q.e.x];
+      q.vj = vin[q.e.x];
+      q.wj = win[q.e.x];
+      q.d = ld_nt(row.gd + (size_t)(q.slot >= 0 ? q.slot : 0) * N + q.so);
+    };
+    auto pair_finish = [&](const int base, const PairIn& q) {
+      const int g = base + lane;
+      const bool has = g < total;
+      const int o = owner_lane(incl, g);
+      const int o_first = __shfl(first, o);
+      const int so = q.so;
       const int o_n = __shfl(nh0, o);
       const int o_added = __shfl(added, o);
       // the owner's position record, from the workgroup's tile
@@ -799,26 +839,12 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
       const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
       const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
-      int2 e = make_int2(0, -1);
-      int slot = -1;
-      double4 pj = make_double4(0., 0., 0., 0.), vj = pj, wj = pj;
-      H4 d{};
-      if (has) {
-        const int r = g - o_first;  // rank among the owner's touching pairs
-        // (the LDS read is marked non-temporal so it stays an LDS instruction)
-        e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
-        if (r >= HIT_STASH) e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
-        slot = row.find_in(wtx + o, so, o_n, e.y);
-        pj = pos_of(e.x);
-        vj = vin[e.x];
-        wj = win[e.x];
-        if (slot >= 0) d = ld_nt(row.gd + (size_t)slot * N + so);
-      }
+      const int2 e = q.e;
+      const int slot = q.slot;
+      const double4 pj = q.pj, vj = q.vj, wj = q.wj;
       // new contacts take their particle's next free slots in list order
       const uint64_t fresh = __ballot(has && slot < 0);
       const uint64_t before_me = lane_range(max(o_first - base, 0), lane);
-      V3<T> fn{T(0), T(0), T(0)}, ft{T(0), T(0), T(0)}, mm{T(0), T(0), T(0)};
-      double delMagD = 0.;
       int wslot = -1;
       if (has) {
         wslot = slot;
@@ -831,38 +857,46 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
             err |= GDEM_ERR_HIST_OVERFLOW_BIT;
           }
         }
-        const double dx = opx - pj.x, dy = opy - pj.y, dz = opz - pj.z;
-        const double distsq = dx * dx + dy * dy + dz * dz;
-        const double R = ori + pj.w;
-        const double dist = sqrt(distsq);
-        delMagD = R - dist;
-        V3<T> n;
-        if constexpr (sizeof(T) == 8) {
-          n = V3<T>{dx / dist, dy / dist, dz / dist};
-        } else {
-          const T inv = T(1) / (T)dist;
-          n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
-        }
-        const T delMag = (T)delMagD;
-        const V3<T> del = scal(-delMag, n);
-        const V3<T> vr{ovi.x - (T)vj.x, ovi.y - (T)vj.y, ovi.z - (T)vj.z};
-        const T vrn = dot(vr, n);
-        const V3<T> vn = scal(vrn, n);
-        const T meff = T(1) / ((T)oinvm + (T)vj.w);
-        const T eta = oetac * sqrt(meff);
-        fn.x = -ok * del.x - eta * vn.x;
-        fn.y = -ok * del.y - eta * vn.y;
-        fn.z = -ok * del.z - eta * vn.z;
-        const T rj = (T)pj.w;
-        V3<T> vrot{oriT * owi.x + rj * (T)wj.x, oriT * owi.y + rj * (T)wj.y, oriT * owi.z + rj * (T)wj.z};
-        vrot = cross(vrot, n);
-        const V3<T> vt = vadd(vsub(vr, vn), vrot);
-        const V3<T> old = slot >= 0 ? V3<T>{(T)d.x, (T)d.y, (T)d.z} : V3<T>{T(0), T(0), T(0)};
-        V3<T> dn;
-        ft = tangential_force<T>(old, fn, vt, n, eta, ok, mu, dtT, dn);
-        if (wslot >= 0) store_delta<H, T>(row.gd + (size_t)wslot * N + so, dn);
-        mm = scal(oriT, cross(n, ft));
       }
+      // The arithmetic runs on every lane (a lane without a pair works on
+      // its owner's own records and its results are never read), so the
+      // memory reads it uses stay unconditional; only the store is guarded.
+      const double dx = opx - pj.x, dy = opy - pj.y, dz = opz - pj.z;
+      const double distsq = dx * dx + dy * dy + dz * dz;
+      const double R = ori + pj.w;
+      const double dist = sqrt(distsq);
+      const double delMagD = R - dist;
+      V3<T> n;
+      if constexpr (sizeof(T) == 8) {
+        n = V3<T>{dx / dist, dy / dist, dz / dist};
+      } else {
+        const T inv = T(1) / (T)dist;
+        n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
+      }
+      const T delMag = (T)delMagD;
+      const V3<T> del = scal(-delMag, n);
+      const V3<T> vr{ovi.x - (T)vj.x, ovi.y - (T)vj.y, ovi.z - (T)vj.z};
+      const T vrn = dot(vr, n);
+      const V3<T> vn = scal(vrn, n);
+      const T meff = T(1) / ((T)oinvm + (T)vj.w);
+      const T eta = oetac * sqrt(meff);
+      V3<T> fn;
+      fn.x = -ok * del.x - eta * vn.x;
+      fn.y = -ok * del.y - eta * vn.y;
+      fn.z = -ok * del.z - eta * vn.z;
+      const T rj = (T)pj.w;
+      V3<T> vrot{oriT * owi.x + rj * (T)wj.x, oriT * owi.y + rj * (T)wj.y, oriT * owi.z + rj * (T)wj.z};
+      vrot = cross(vrot, n);
+      const V3<T> vt = vadd(vsub(vr, vn), vrot);
+      const V3<T> old = slot >= 0 ? V3<T>{(T)q.d.x, (T)q.d.y, (T)q.d.z} : V3<T>{T(0), T(0), T(0)};
+      V3<T> dn;
+      const V3<T> ft = tangential_force<T>(old, fn, vt, n, eta, ok, mu, dtT, dn);
+      if (has && wslot >= 0) store_delta<H, T>(row.gd + (size_t)wslot * N + so, dn);
+      const V3<T> mm = scal(oriT, cross(n, ft));
+      // (keeps the unused fourth word's register reserved until the load
+      // that writes it has landed, instead of reusing it at once and then
+      // waiting for every load in flight)
+      asm volatile("" ::"v"(q.d.w));
       // every particle adds its pairs of this round in list order
       const int my_lo = max(first - base, 0), my_hi = min(first + cnt - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
@@ -905,6 +939,12 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         }
       }
       added += __popcll(fresh & lane_range(my_lo, my_hi));
+    };
+    for (int base = 0; base < total; base += 64) {
+      PairIn q;
+      pair_key(base, q);
+      pair_data(q);
+      pair_finish(base, q);
     }
     row.n = min(nh0 + added, row.cap);
     row.marks = marks;
