@@ -255,6 +255,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
     A.nwl = B.nwl;
     A.tri = B.tri;
     A.N = B.cap;
+    A.nb_cap = B.nb_cap;
     A.hist_cap = B.hist_cap;
     A.whist_cap = B.whist_cap;
     A.dt = S.dt;
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
     }
     A.threshSq = S.threshSq;
   }
-  const Own<H> own = load_own(A, s, running);
+  const Own<H> own = load_own<false>(A, s, running);
   tile_p[threadIdx.x] = own.p;
   __syncthreads();
   if (sid >= 0) {
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
     SlotCtl& C = B.sctl[sid];
     if (running) {
       Rec<T> none{};
-      step_particle<T, H, false, PRow, WRow>(A, s, true, own, tile_p, s - (int)threadIdx.x, lds, none, err,
+      step_particle<T, H, false, false, PRow, WRow>(A, s, true, own, tile_p, s - (int)threadIdx.x, lds, none, err,
                                              refresh);
       // a refresh at the scene's last step would only move its final state
       if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {

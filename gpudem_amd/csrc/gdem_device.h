@@ -551,22 +551,33 @@ struct StepArgs {
   const int* nwl;
   const TriGeo* tri;
   int N;
-  int hist_cap, whist_cap;
+  int nb_cap, hist_cap, whist_cap;
   double dt, mu;
   double g[3], bmin[3], bmax[3];
   double threshSq;
 };
 
-// A particle's own records: state, material, list and history lengths.  All loads are independent, so they share one memory round
-// trip; lanes without a particle load nothing.
+// pair-list entries screened per batch
+constexpr int SCREEN_BATCH = 8;
+
+// A particle's own records: state, material, list and history lengths, and
+// (PRE) the first SCREEN_BATCH pair-list entries.  All loads are
+// independent, so they share one memory round trip; lanes without a particle
+// load nothing.  The entries are fetched before the list length is known:
+// rows past the length (always inside the [nb_cap][N] array) hold stale
+// entries, which the screening pass masks.  This saves the round trip that
+// waits for the length, at the price of reading SCREEN_BATCH rows whatever
+// the length: a gain for dense packings (lists of about eight and longer),
+// a loss for dilute ones.
 template <class H>
 struct Own {
   double4 p, v, w, rf;  // rf (reference position) only in the mixed form
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
+  int2 nb0[SCREEN_BATCH];
 };
 
-template <class H>
+template <bool PRE, class H>
 __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool valid) {
   Own<H> o;
   o.p = make_double4(0., 0., 0., -1.);
@@ -575,6 +586,8 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.rf = o.p;
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
+#pragma unroll
+  for (int u = 0; u < SCREEN_BATCH; ++u) o.nb0[u] = make_int2(-1, 0);
   if (valid) {
     o.p = A.pin[s];
     o.v = A.vin[s];
@@ -585,6 +598,14 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     o.nwl = ld_nt(A.nwl + s);
     o.nh0 = ld_nt(A.nh + s);
     o.nw0 = ld_nt(A.nw + s);
+    // (rows clamped rather than skipped: a branch here would leave the
+    // compiler unable to count the loads in flight, and it would then wait
+    // for all of them before the next dependent access)
+    if constexpr (PRE) {
+#pragma unroll
+      for (int u = 0; u < SCREEN_BATCH; ++u)
+        o.nb0[u] = A.nbr[(size_t)(u < A.nb_cap ? u : A.nb_cap - 1) * A.N + s];
+    }
   }
   return o;
 }
@@ -668,7 +689,7 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // slots [tile0, tile0 + 256), this workgroup's particles, so neighbours inside
 // the tile (most of them, in Morton order) are read from LDS instead of
 // memory.
-template <class T, class H, bool REC, class PRow, class WRow, class Lds>
+template <class T, class H, bool REC, bool PRE, class PRow, class WRow, class Lds>
 __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, const Own<H>& own,
                                               const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
                                               uint32_t& err, bool& refresh) {
@@ -700,6 +721,17 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const H4 mt = own.mt;
   const int nnb = act ? own.nnb : 0, nwl = act ? own.nwl : 0;
   const int nh0 = act ? own.nh0 : 0, nw0 = act ? own.nw0 : 0;
+  if constexpr (PRE) {
+    // The prefetched list entries arrive with the records above and are
+    // waited for here (the empty asm uses them): left to the compiler, the
+    // wait for the last entry would come after the wall-list and candidate
+    // position loads below, and would wait for those too.
+    static_assert(SCREEN_BATCH == 8, "the wait below names eight entries");
+    asm volatile("" ::"v"(own.nb0[0].x), "v"(own.nb0[0].y), "v"(own.nb0[1].x), "v"(own.nb0[1].y),
+                 "v"(own.nb0[2].x), "v"(own.nb0[2].y), "v"(own.nb0[3].x), "v"(own.nb0[3].y));
+    asm volatile("" ::"v"(own.nb0[4].x), "v"(own.nb0[4].y), "v"(own.nb0[5].x), "v"(own.nb0[5].y),
+                 "v"(own.nb0[6].x), "v"(own.nb0[6].y), "v"(own.nb0[7].x), "v"(own.nb0[7].y));
+  }
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
   const double invmoi = (double)mt.w;
   const double ri = p.w, invm = v.w;
@@ -723,6 +755,20 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     if (lane < w_total) w_t0 = ld_nt(A.wl + (size_t)(lane - o_first) * N + so);
   }
 
+  // (PRE) The first batch of pair candidates came with the particle's own
+  // records; their positions are requested now, before the history keys
+  // below, because any LDS access after those loads (these tile reads
+  // included) waits for them.
+  int2 ev0[SCREEN_BATCH];
+  double4 cv0[SCREEN_BATCH];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int u = 0; u < SCREEN_BATCH; ++u) ev0[u] = u < nnb ? own.nb0[u] : make_int2(-1, 0);
+#pragma unroll
+    for (int u = 0; u < SCREEN_BATCH; ++u)
+      cv0[u] = ev0[u].x >= 0 ? pos_of(ev0[u].x) : make_double4(0., 0., 0., -1.);
+  }
+
   // history rows of pairs and walls
   PRow prow;
   bind_lds(prow, &lds.pkey[0][0], tx);
@@ -738,9 +784,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   wrow.s = s;
   wrow.N = N;
   wrow.cap = A.whist_cap;
-  // (after the wave shuffles above: an LDS access after the loads below waits
-  // for them, and the first one is the screening pass's, which waits for its
-  // own candidate loads anyway)
+  // (after the wave shuffles and tile reads above: an LDS access after the
+  // loads below waits for them, and the first one is the screening pass's,
+  // which waits for its own candidate loads anyway)
   load_rows(prow, nh0, wrow, nw0, act);
   // ---- particle pairs -------------------------------------------------------
   {
@@ -750,15 +796,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     uint64_t hits = 0;
     int nhit = 0;
     int2* stash = &lds.hit[0][tx];
-    for (int q0 = 0; q0 < nnb; q0 += 8) {
-      int2 ev[8];
-      double4 cv[8];
+    auto test = [&](const int q0, const int2* ev, const double4* cv) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : make_int2(-1, 0);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) cv[u] = ev[u].x >= 0 ? pos_of(ev[u].x) : make_double4(0., 0., 0., -1.);
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < SCREEN_BATCH; ++u) {
         const double4 pj = cv[u];
         if (pj.w < 0.) continue;
         const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
@@ -770,6 +810,23 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           ++nhit;
         }
       }
+    };
+    int q_start = 0;
+    if constexpr (PRE) {
+      // (outside the loop, so that no wait for a later batch's loads precedes it)
+      test(0, ev0, cv0);
+      q_start = SCREEN_BATCH;
+    }
+    for (int q0 = q_start; q0 < nnb; q0 += SCREEN_BATCH) {
+      int2 ev[SCREEN_BATCH];
+      double4 cv[SCREEN_BATCH];
+#pragma unroll
+      for (int u = 0; u < SCREEN_BATCH; ++u)
+        ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : make_int2(-1, 0);
+#pragma unroll
+      for (int u = 0; u < SCREEN_BATCH; ++u)
+        cv[u] = ev[u].x >= 0 ? pos_of(ev[u].x) : make_double4(0., 0., 0., -1.);
+      test(q0, ev, cv);
     }
     // the entries are read from other lanes' columns below
     __builtin_amdgcn_wave_barrier();

@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   A.nwl = B.nwl;
   A.tri = B.tri;
   A.N = P.N;
+  A.nb_cap = P.nb_cap;
   A.hist_cap = P.hist_cap;
   A.whist_cap = P.whist_cap;
   A.dt = P.dt;
@@ -57,12 +58,12 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   // this workgroup's positions, for in-tile neighbour reads
   __shared__ double4 tile_p[256];
   const bool valid = s < P.N;
-  const Own<H> own = load_own(A, s, valid);
+  const Own<H> own = load_own<true>(A, s, valid);
   tile_p[threadIdx.x] = own.p;
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
-  step_particle<T, H, REC, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
+  step_particle<T, H, REC, true, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
   if (err) atomicOr((unsigned int*)&B.ctl[CTL_ERR], err);
   if (last_block(&B.ctl[CTL_DONE_STEP])) {
