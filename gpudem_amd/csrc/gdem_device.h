@@ -198,29 +198,37 @@ __device__ __forceinline__ TriHit tri_distance(const TriGeo* __restrict__ tg, do
 // The same test in two memory round trips, for the contact pass.  Reading the
 // fields as tri_distance_g needs them costs up to four dependent trips
 // (plane; barycentric terms; region data; feature id).  Here the first trip
-// requests the plane, the first vertex, both edges from it and the
-// barycentric terms together (an empty asm uses them at once, so the
-// compiler cannot sink the later ones into the branches after the plane
-// test); a point outside the face region then requests its region's data in
-// one second trip: the base point, edge, inverse squared length and feature
+// (tri_first) requests the plane, the first vertex, both edges from it and
+// the barycentric terms together; a point outside the face region then
+// requests its region's data in one second trip: the base point, edge, inverse squared length and feature
 // id, at offsets selected per lane.  The arithmetic is tri_distance_g's,
 // expression for expression: the edge parameter dot(pos - P, E) * inv equals
 // its d20 * d00inv and d21 * d11inv for the edges from the first vertex.
-__device__ __forceinline__ TriHit tri_distance_2trip(const TriGeo* __restrict__ g, double px, double py,
-                                                     double pz, double Rsq) {
+// The first trip's fields.
+struct TriFirst {
+  V3<double> nt, P0, E01, E02;
+  double pd, d00, d01, d11, den;
+};
+__device__ __forceinline__ TriFirst tri_first(const TriGeo* __restrict__ g) {
+  TriFirst f;
+  f.nt = V3<double>{g->n[0], g->n[1], g->n[2]};
+  f.pd = g->d;
+  f.P0 = V3<double>{g->p0[0], g->p0[1], g->p0[2]};
+  f.E01 = V3<double>{g->e01[0], g->e01[1], g->e01[2]};
+  f.E02 = V3<double>{g->e02[0], g->e02[1], g->e02[2]};
+  f.d00 = g->d00, f.d01 = g->d01, f.d11 = g->d11, f.den = g->denom;
+  return f;
+}
+// The test given the first trip's fields (tri_first, issued by the caller
+// before work that does not need them).
+__device__ __forceinline__ TriHit tri_distance_2trip(const TriFirst& f, const TriGeo* __restrict__ g, double px,
+                                                     double py, double pz, double Rsq) {
   TriHit res;
   res.n = V3<double>{0., 0., 0.};
   res.dist = 1.e10;
   res.hit = -1;
-  const V3<double> nt{g->n[0], g->n[1], g->n[2]};
-  const double pd = g->d;
-  const V3<double> P0{g->p0[0], g->p0[1], g->p0[2]};
-  const V3<double> E01{g->e01[0], g->e01[1], g->e01[2]};
-  const V3<double> E02{g->e02[0], g->e02[1], g->e02[2]};
-  const double d00 = g->d00, d01 = g->d01, d11 = g->d11, den = g->denom;
-  asm volatile("" ::"v"(nt.x), "v"(nt.y), "v"(nt.z), "v"(pd), "v"(P0.x), "v"(P0.y), "v"(P0.z));
-  asm volatile("" ::"v"(E01.x), "v"(E01.y), "v"(E01.z), "v"(E02.x), "v"(E02.y), "v"(E02.z));
-  asm volatile("" ::"v"(d00), "v"(d01), "v"(d11), "v"(den));
+  const V3<double> nt = f.nt, P0 = f.P0, E01 = f.E01, E02 = f.E02;
+  const double pd = f.pd, d00 = f.d00, d01 = f.d01, d11 = f.d11, den = f.den;
   double dist = 0.;
   dist += nt.x * px;
   dist += nt.y * py;
@@ -849,13 +857,17 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // cannot count the reads in flight and waits for all of them, the
     // previous round's history stores included.
     struct PairIn {
-      int2 e;    // (partner slot, partner id)
-      int slot;  // history slot, -1 for a new contact
-      int so;    // owner's slot
+      int2 e;       // (partner slot, partner id)
+      int slot;     // history slot, -1 for a new contact
+      int so;       // owner's slot
+      int o, o_first, o_n;  // owner's lane, its first pair's index, its history length
       double4 pj, vj, wj;
       H4 d;
     };
-    auto pair_key = [&](const int base, PairIn& q) {
+    // The partner's records are requested as soon as the list entry is
+    // known, before the history key search (dependent LDS reads), which only
+    // the displacement read needs.
+    auto pair_fetch = [&](const int base, PairIn& q) {
       const int g = base + lane;
       const bool has = g < total;
       const int o = owner_lane(incl, g);
@@ -872,22 +884,25 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         asm volatile("" ::"v"(e.x), "v"(e.y));
       }
       q.so = so;
+      q.o = o;
+      q.o_first = o_first;
+      q.o_n = o_n;
       q.e = has ? e : make_int2(so, -1);
-      q.slot = has ? row.find_in(wtx + o, so, o_n, e.y) : -1;
-    };
-    auto pair_data = [&](PairIn& q) {
       q.pj = pin[q.e.x];
       q.vj = vin[q.e.x];
       q.wj = win[q.e.x];
+      // (keeps the compiler from sinking the loads above below the search)
+      asm volatile("" ::: "memory");
+      q.slot = has ? row.find_in(wtx + o, so, o_n, e.y) : -1;
       q.d = ld_nt(row.gd + (size_t)(q.slot >= 0 ? q.slot : 0) * N + q.so);
     };
     auto pair_finish = [&](const int base, const PairIn& q) {
       const int g = base + lane;
       const bool has = g < total;
-      const int o = owner_lane(incl, g);
-      const int o_first = __shfl(first, o);
+      const int o = q.o;
+      const int o_first = q.o_first;
       const int so = q.so;
-      const int o_n = __shfl(nh0, o);
+      const int o_n = q.o_n;
       const int o_added = __shfl(added, o);
       // the owner's position record, from the workgroup's tile
       const double4 op = ld_nt(tile_p + wtx + o);
@@ -999,8 +1014,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     };
     for (int base = 0; base < total; base += 64) {
       PairIn q;
-      pair_key(base, q);
-      pair_data(q);
+      pair_fetch(base, q);
       pair_finish(base, q);
     }
     row.n = min(nh0 + added, row.cap);
@@ -1044,12 +1058,16 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       if (has) {
         // candidate, its history slot and displacement, and its distance
         t = t_in;
+        // The triangle's first fields are requested before the history key
+        // search (dependent LDS reads); the empty asm keeps the compiler
+        // from sinking the loads below it.  The displacement (slot row 0,
+        // always allocated, stands in when there is none, so the load needs
+        // no branch) follows the search.
+        const TriFirst tf = tri_first(A.tri + t);
+        asm volatile("" ::: "memory");
         slot = row.find_in(wtx + o, so, o_n, t);
-        // The history displacement (slot row 0, always allocated, stands in
-        // when there is none, so the load needs no branch) is requested with
-        // the triangle's first fields.
         const H4 d = ld_nt(row.gd + (size_t)(slot >= 0 ? slot : 0) * N + so);
-        const TriHit th = tri_distance_2trip(A.tri + t, opx, opy, opz, orsq);
+        const TriHit th = tri_distance_2trip(tf, A.tri + t, opx, opy, opz, orsq);
         if (slot >= 0) old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
         dist = th.dist;
         hit = th.hit;
