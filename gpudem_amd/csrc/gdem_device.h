@@ -873,7 +873,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
       const uint64_t o_hits = shfl64(hits, o);
-      const int so = __shfl(s, o);
+      // A lane without a pair uses the slot of lane 0, which holds a particle
+      // whenever the wave has pairs.  (The shuffle runs on every lane: in a
+      // branch it would read lanes that are off in it, and get nothing.)
+      const int so_o = __shfl(s, o);
+      const int so = has ? so_o : s - lane;
       const int o_n = __shfl(nh0, o);
       const int r = g - o_first;  // rank among the owner's touching pairs
       // (the LDS read is marked non-temporal so it stays an LDS instruction)
