@@ -74,6 +74,7 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
     HIPCHK(c, A.alloc(&B.idx[h], N));
   }
   HIPCHK(c, A.alloc(&B.ref, N));
+  HIPCHK(c, A.alloc(&B.ref32, N));
   HIPCHK(c, A.alloc(&B.nbr, N * sc->P.nb_cap));
   HIPCHK(c, A.alloc(&B.nnb, N));
   HIPCHK(c, A.alloc(&B.wl, N * sc->P.wall_cap));

@@ -187,6 +187,7 @@ int alloc_batch(gdem_batch* b) {
     HIPCHK(c, hipMemsetAsync(B.pid[h], 0xff, N * 4, c->stream));
   }
   HIPCHK(c, A.alloc(&B.ref, N));
+  HIPCHK(c, A.alloc(&B.ref32, N));
   HIPCHK(c, A.alloc(&B.nbr, N * b->nb_cap));
   HIPCHK(c, A.alloc(&B.nnb, N));
   HIPCHK(c, A.alloc(&B.wl, N * b->wall_cap));

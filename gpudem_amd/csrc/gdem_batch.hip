@@ -192,6 +192,7 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   L.nbr = B.nbr;
   L.nnb = B.nnb;
   L.ref = B.ref;
+  L.ref32 = B.ref32;
   L.wl = B.wl;
   L.nwl = B.nwl;
   L.tri = B.tri;
@@ -249,6 +250,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
     A.wd = B.wd[q];
     A.nw = B.nw[q];
     A.ref = B.ref;
+    A.ref32 = B.ref32;
     A.nbr = B.nbr;
     A.nnb = B.nnb;
     A.wl = B.wl;

@@ -553,6 +553,7 @@ struct StepArgs {
   H4* wd;
   int* nw;
   const double4* ref;
+  const float4* ref32;
   const int2* nbr;
   const int* nnb;
   const int* wl;
@@ -579,7 +580,8 @@ constexpr int SCREEN_BATCH = 8;
 // a loss for dilute ones.
 template <class H>
 struct Own {
-  double4 p, v, w, rf;  // rf (reference position) only in the mixed form
+  double4 p, v, w;
+  float4 rf32;  // the reference position rounded to float (mixed form only)
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
   int2 nb0[SCREEN_BATCH];
@@ -591,7 +593,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.p = make_double4(0., 0., 0., -1.);
   o.v = make_double4(0., 0., 0., 0.);
   o.w = o.v;
-  o.rf = o.p;
+  o.rf32 = make_float4(0.f, 0.f, 0.f, 0.f);
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
 #pragma unroll
@@ -601,7 +603,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     o.v = A.vin[s];
     o.w = A.win[s];
     o.mt = ld_nt(A.mat + s);
-    if constexpr (sizeof(H) != 8) o.rf = ld_nt(A.ref + s);
+    if constexpr (sizeof(H) != 8) o.rf32 = ld_nt(A.ref32 + s);
     o.nnb = ld_nt(A.nnb + s);
     o.nwl = ld_nt(A.nwl + s);
     o.nh0 = ld_nt(A.nh + s);
@@ -616,6 +618,30 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     }
   }
   return o;
+}
+
+// The reference's refresh test (main.cu, cpu_dem.cu:1529-1552):
+// |x - ref|^2 > thresh in FP64, with ref the position at the last list
+// build.  It is first decided on ref rounded to float (16 B read per step
+// instead of 32): each rounded component is within half a float ulp,
+// |r| * 2^-24, of the exact one, so the distance from the rounded point is
+// within m = 2^-24 * (|r.x| + |r.y| + |r.z|) of the exact distance.  Outside
+// a band of twice that width around the threshold distance (plus a margin
+// for the double rounding of the distance itself) the two tests agree; in
+// it (rarely) the exact position is read and the reference's expression is
+// evaluated as is.
+__device__ __forceinline__ bool refresh_test(const double4& po, const float4 r32, const double4* ref,
+                                             double threshSq) {
+  const double ex = po.x - (double)r32.x, ey = po.y - (double)r32.y, ez = po.z - (double)r32.z;
+  const double d = sqrt(ex * ex + ey * ey + ez * ez);
+  const double t = sqrt(threshSq);
+  const double m = 2.0 * 5.9604644775390625e-8 * (fabs((double)r32.x) + fabs((double)r32.y) + fabs((double)r32.z)) +
+                   1e-12 * (d + t) + 1e-38;
+  if (d > t + m) return true;
+  if (d < t - m) return false;
+  const double4 rf = ld_nt(ref);
+  const double fx = po.x - rf.x, fy = po.y - rf.y, fz = po.z - rf.z;
+  return (fx * fx + fy * fy + fz * fz) > threshSq;
 }
 
 // ---- wave-level helpers (64 lanes) ------------------------------------------
@@ -1205,18 +1231,19 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   // spill at its occupancy bound; the empty asm keeps the compiler from
   // reusing the values loaded at the start.  The mixed form keeps them (one
   // round trip fewer, measured faster).
-  double4 p1, v1, w1, rf;  // rf: position at the last list build
+  double4 p1, v1, w1;
+  float4 rf32;  // position at the last list build, rounded to float
   if constexpr (sizeof(T) == 8) {
     asm volatile("" ::: "memory");
     p1 = ld_nt(tile_p + tx);
     v1 = vin[s];
     w1 = win[s];
-    rf = ld_nt(A.ref + s);
+    rf32 = ld_nt(A.ref32 + s);
   } else {
     p1 = p;
     v1 = v;
     w1 = w;
-    rf = own.rf;
+    rf32 = own.rf32;
   }
   const double dt = A.dt;
   const double ax = (double)f.x * invm + A.g[0];
@@ -1252,10 +1279,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const bool out = po.x < A.bmin[0] || po.x > A.bmax[0] || po.y < A.bmin[1] || po.y > A.bmax[1] ||
                    po.z < A.bmin[2] || po.z > A.bmax[2];
   po.w = out ? -p1.w : p1.w;
-  if (!out) {
-    const double ex = po.x - rf.x, ey = po.y - rf.y, ez = po.z - rf.z;
-    refresh = (ex * ex + ey * ey + ez * ez) > A.threshSq;
-  }
+  if (!out) refresh = refresh_test(po, rf32, A.ref + s, A.threshSq);
   if (!(isfinite(po.x) && isfinite(po.y) && isfinite(po.z) && isfinite(vo.x) && isfinite(vo.y) &&
         isfinite(vo.z)))
     err |= GDEM_ERR_NAN_BIT;
@@ -1274,6 +1298,7 @@ struct ListArgs {
   int2* nbr;
   int* nnb;
   double4* ref;
+  float4* ref32;
   int* wl;
   int* nwl;
   const TriGeo* tri;
@@ -1428,7 +1453,11 @@ __device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
 
 // lds: the workgroup's PairLds (threads of a PL_BLOCK-thread workgroup)
 __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active, PairLds& lds) {
-  L.ref[s] = L.pos[s];
+  {
+    const double4 p = L.pos[s];
+    L.ref[s] = p;
+    L.ref32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+  }
   if (!active) {
     L.nnb[s] = 0;
     return;

@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   A.wd = B.wd[hc];
   A.nw = B.nw[hc];
   A.ref = B.ref;
+  A.ref32 = B.ref32;
   A.nbr = B.nbr;
   A.nnb = B.nnb;
   A.wl = B.wl;
@@ -387,6 +388,7 @@ __device__ __forceinline__ ListArgs list_args(const SimParams& P, const Bufs<H>&
   L.nbr = B.nbr;
   L.nnb = B.nnb;
   L.ref = B.ref;
+  L.ref32 = B.ref32;
   L.wl = B.wl;
   L.nwl = B.nwl;
   L.tri = B.tri;

@@ -94,6 +94,7 @@ struct Bufs {
   H4* wd[2];         // [whist_cap][N]
   int* nw[2];
   double4* ref;      // positions at the last list build
+  float4* ref32;     // the same rounded to float (the refresh test's first decision)
   int2* nbr;         // pair candidates (internal index, original id), slot-major [nb_cap][N]
   int* nnb;
   int* wl;           // wall candidates (triangle id) [wall_cap][N]
@@ -173,6 +174,7 @@ struct BBufs {
   H4* wd[2];
   int* nw[2];
   double4* ref;
+  float4* ref32;
   int2* nbr;
   int* nnb;
   int* wl;
