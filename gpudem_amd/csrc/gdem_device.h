@@ -83,6 +83,20 @@ __device__ __forceinline__ float4 ld_nt(const float4* p) {
   const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
   return make_float4(v[0], v[1], v[2], v[3]);
 }
+template <class H>
+__device__ __forceinline__ Del3<H> ld_nt(const Del3<H>* p) {
+  Del3<H> r;
+  r.x = __builtin_nontemporal_load(&p->x);
+  r.y = __builtin_nontemporal_load(&p->y);
+  r.z = __builtin_nontemporal_load(&p->z);
+  return r;
+}
+template <class H>
+__device__ __forceinline__ void st_nt(Del3<H> a, Del3<H>* p) {
+  __builtin_nontemporal_store(a.x, &p->x);
+  __builtin_nontemporal_store(a.y, &p->y);
+  __builtin_nontemporal_store(a.z, &p->z);
+}
 __device__ __forceinline__ void st_nt(double4 a, double4* p) {
   typedef double v4 __attribute__((ext_vector_type(4)));
   const v4 v = {a.x, a.y, a.z, a.w};
@@ -311,7 +325,7 @@ template <class H, int CAP>
 struct HistLds {
   using H4 = typename Vec4<H>::type;
   int* gkey;
-  H4* gd;
+  Del3<H>* gd;
   int s, N, cap;
   int n;           // live slots
   uint64_t marks;  // slots in contact this step
@@ -353,7 +367,7 @@ template <class H>
 struct HistScan {
   using H4 = typename Vec4<H>::type;
   int* gkey;
-  H4* gd;
+  Del3<H>* gd;
   int s, N, cap;
   int n;
   uint64_t marks;
@@ -414,12 +428,11 @@ __device__ __forceinline__ V3<T> tangential_force(V3<T> old, V3<T> fn, V3<T> vt,
 }
 
 template <class H, class T>
-__device__ __forceinline__ void store_delta(typename Vec4<H>::type* p, V3<T> dn) {
-  typename Vec4<H>::type o;
-  o.x = (decltype(o.x))dn.x;
-  o.y = (decltype(o.y))dn.y;
-  o.z = (decltype(o.z))dn.z;
-  o.w = 0;
+__device__ __forceinline__ void store_delta(Del3<H>* p, V3<T> dn) {
+  Del3<H> o;
+  o.x = (H)dn.x;
+  o.y = (H)dn.y;
+  o.z = (H)dn.z;
   st_nt(o, p);
 }
 
@@ -547,10 +560,10 @@ struct StepArgs {
   double4* wout;
   const H4* mat;
   int* hkey;
-  H4* hd;
+  Del3<H>* hd;
   int* nh;
   int* wkey;
-  H4* wd;
+  Del3<H>* wd;
   int* nw;
   const double4* ref;
   const float4* ref32;
@@ -888,7 +901,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       int so;       // owner's slot
       int o, o_first, o_n;  // owner's lane, its first pair's index, its history length
       double4 pj, vj, wj;
-      H4 d;
+      Del3<H> d;
     };
     // The partner's records are requested as soon as the list entry is
     // known, before the history key search (dependent LDS reads), which only
@@ -995,10 +1008,6 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const V3<T> ft = tangential_force<T>(old, fn, vt, n, eta, ok, mu, dtT, dn);
       if (has && wslot >= 0) store_delta<H, T>(row.gd + (size_t)wslot * N + so, dn);
       const V3<T> mm = scal(oriT, cross(n, ft));
-      // (keeps the unused fourth word's register reserved until the load
-      // that writes it has landed, instead of reusing it at once and then
-      // waiting for every load in flight)
-      asm volatile("" ::"v"(q.d.w));
       // every particle adds its pairs of this round in list order
       const int my_lo = max(first - base, 0), my_hi = min(first + cnt - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
@@ -1096,7 +1105,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         const TriFirst tf = tri_first(A.tri + t);
         asm volatile("" ::: "memory");
         slot = row.find_in(wtx + o, so, o_n, t);
-        const H4 d = ld_nt(row.gd + (size_t)(slot >= 0 ? slot : 0) * N + so);
+        const Del3<H> d = ld_nt(row.gd + (size_t)(slot >= 0 ? slot : 0) * N + so);
         const TriHit th = tri_distance_2trip(tf, A.tri + t, opx, opy, opz, orsq);
         if (slot >= 0) old = V3<T>{(T)d.x, (T)d.y, (T)d.z};
         dist = th.dist;

@@ -634,11 +634,10 @@ __global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, His
   for (int q = 0; q < nh; ++q) {
     const size_t src = (size_t)id * P.hist_cap + q;
     B.hkey[hc][(size_t)q * N + s] = in.hist_key[src];
-    H4 d;
+    Del3<H> d;
     d.x = (H)in.hist_delta[src * 3];
     d.y = (H)in.hist_delta[src * 3 + 1];
     d.z = (H)in.hist_delta[src * 3 + 2];
-    d.w = 0;
     B.hd[hc][(size_t)q * N + s] = d;
   }
   int nw = in.nwhist[id];
@@ -650,11 +649,10 @@ __global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, His
   for (int q = 0; q < nw; ++q) {
     const size_t src = (size_t)id * P.whist_cap + q;
     B.wkey[hc][(size_t)q * N + s] = in.whist_key[src];
-    H4 d;
+    Del3<H> d;
     d.x = (H)in.whist_delta[src * 3];
     d.y = (H)in.whist_delta[src * 3 + 1];
     d.z = (H)in.whist_delta[src * 3 + 2];
-    d.w = 0;
     B.wd[hc][(size_t)q * N + s] = d;
   }
 }

@@ -77,6 +77,13 @@ struct Vec4<float> {
   using type = float4;
 };
 
+// A tangential history displacement, packed (12 B in the mixed form, 24 B in
+// FP64): a wave's slot row is read and written as one contiguous run.
+template <class H>
+struct Del3 {
+  H x, y, z;
+};
+
 // All device buffers of a scene.  Passed by value; which half of a [2]
 // array is current is read from the control words at kernel start.
 template <class H>
@@ -88,10 +95,10 @@ struct Bufs {
   H4* mat[2];        // k, etaconst, sqrt(m), 1/I (FP64, or FP32 in the mixed form)
   int* pid[2];       // original particle id
   int* hkey[2];      // pair history keys, slot-major [hist_cap][N]
-  H4* hd[2];         // pair history deltas (x, y, z, unused), slot-major [hist_cap][N]
+  Del3<H>* hd[2];    // pair history deltas, slot-major [hist_cap][N]
   int* nh[2];
   int* wkey[2];      // wall history keys [whist_cap][N]
-  H4* wd[2];         // [whist_cap][N]
+  Del3<H>* wd[2];    // [whist_cap][N]
   int* nw[2];
   double4* ref;      // positions at the last list build
   float4* ref32;     // the same rounded to float (the refresh test's first decision)
@@ -168,10 +175,10 @@ struct BBufs {
   H4* mat[2];
   int* pid[2];  // id within the scene, -1 for padding
   int* hkey[2];
-  H4* hd[2];
+  Del3<H>* hd[2];
   int* nh[2];
   int* wkey[2];
-  H4* wd[2];
+  Del3<H>* wd[2];
   int* nw[2];
   double4* ref;
   float4* ref32;
