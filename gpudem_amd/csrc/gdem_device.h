@@ -493,11 +493,11 @@ constexpr int step_waves() {
 
 // Touching pairs of a particle whose list entries the screening pass keeps in
 // LDS for the force pass (later ones are read again from the list).
-constexpr int HIT_STASH = 4;
+constexpr int HIT_STASH = 8;
 
 // The hit stash is used only in the pair pass and the vertex/edge features
-// only in the wall pass, so they share their LDS: a thread's eight feature
-// words are the two halves of its own four stash entries (ve_slot), so no
+// only in the wall pass, so they share their LDS: a thread's VE_MAX feature
+// words are the two halves of its own HIT_STASH stash entries (ve_slot), so no
 // thread's features overlap another thread's stash, whatever pass the other
 // waves of the workgroup are in.
 constexpr int VE_MAX = 2 * HIT_STASH;

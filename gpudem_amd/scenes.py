@@ -158,14 +158,14 @@ def helical_chute_mesh(seed=3, turns=4, radius=0.2, pitch=0.1, width=0.05, ntri=
 
 
 def helical_chute(seed=3, n=1000000, r0=0.0005, spread=0.1, density=2500., young=1e7, cor=0.5,
-                  mu=0.5, dt=2e-6, steps=10000, skin_factor=0.3, ntri=20000):
+                  mu=0.5, dt=2e-6, steps=10000, skin_factor=0.25, ntri=20000):
     """1,000,000 spheres (r ~ U[0.9, 1.1] x 0.5 mm) resting in layers along a
     4-turn helical chute of radius 0.2 m, pitch 0.1 m and width 0.05 m meshed as
     20,000 triangles.  Particles sit on a jittered face-centred lattice that
     follows the ramp, a few layers deep, so they settle and flow down the
-    chute together.  The Verlet skin (0.3 r_max, refresh at skin/2) was picked
-    by timing 0.1, 0.2, 0.3 and 0.5 r_max on the settled bed; the skin never
-    changes a trajectory, only how often the lists are rebuilt."""
+    chute together.  The Verlet skin (0.25 r_max, refresh at skin/2) was picked
+    by timing 0.1, 0.15, 0.2, 0.25 and 0.3 r_max on the settled bed; the skin
+    never changes a trajectory, only how often the lists are rebuilt."""
     rng = np.random.default_rng(seed)
     verts, tilt, z_floor = helical_chute_mesh(seed=seed, ntri=ntri)
     rmax = r0 * (1 + spread)
