@@ -323,7 +323,6 @@ __device__ __forceinline__ TriHit tri_distance_2trip(const TriFirst& f, const Tr
 // searches the keys in memory and serves capacities beyond the LDS budget.
 template <class H, int CAP>
 struct HistLds {
-  using H4 = typename Vec4<H>::type;
   int* gkey;
   Del3<H>* gd;
   int s, N, cap;
@@ -365,7 +364,6 @@ struct HistLds {
 
 template <class H>
 struct HistScan {
-  using H4 = typename Vec4<H>::type;
   int* gkey;
   Del3<H>* gd;
   int s, N, cap;
@@ -720,7 +718,8 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // Contact forces, history update, integration, out-of-bounds and refresh test
 // of the particle in slot s, one thread per particle; every lane of the wave
 // calls it (valid: the lane holds a particle).  PRow/WRow are the history row
-// types for pairs and walls.
+// types for pairs and walls; PRE: own holds the first batch of list entries
+// (load_own<true>), REC: every contact is also written to rec.
 //
 // Pair candidates are screened per particle (indices and positions fetched
 // eight at a time, FP64 overlap test).  The wave's touching pairs are then
