@@ -74,7 +74,6 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
     HIPCHK(c, A.alloc(&B.idx[h], N));
   }
   HIPCHK(c, A.alloc(&B.ref, N));
-  HIPCHK(c, A.alloc(&B.ref32, N));
   HIPCHK(c, A.alloc(&B.nbr, N * sc->P.nb_cap));
   HIPCHK(c, A.alloc(&B.nnb, N));
   HIPCHK(c, A.alloc(&B.wl, N * sc->P.wall_cap));
@@ -219,7 +218,7 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
     const double* w = pd->angv ? pd->angv + 3 * i : nullptr;
     // ParticleSystem.cu:294-335: invm = 1/m, sqrtm = sqrt(m), moi = 2/5 m r^2
     vel[i] = make_double4(v ? v[0] : 0., v ? v[1] : 0., v ? v[2] : 0., 1. / m);
-    angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., (double)i);
+    angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., 0.);
     const double rsq = r * r;
     const double moi = 2. / 5. * m * rsq;
     const double mrow[4] = {pd->k[i], pd->etaconst[i], sqrt(m), 1. / moi};

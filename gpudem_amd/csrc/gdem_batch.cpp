@@ -187,7 +187,6 @@ int alloc_batch(gdem_batch* b) {
     HIPCHK(c, hipMemsetAsync(B.pid[h], 0xff, N * 4, c->stream));
   }
   HIPCHK(c, A.alloc(&B.ref, N));
-  HIPCHK(c, A.alloc(&B.ref32, N));
   HIPCHK(c, A.alloc(&B.nbr, N * b->nb_cap));
   HIPCHK(c, A.alloc(&B.nnb, N));
   HIPCHK(c, A.alloc(&B.wl, N * b->wall_cap));
@@ -229,7 +228,7 @@ int upload_particles(gdem_batch* b, const Slot& sl, const gdem_particles_desc* p
     if (i >= pd->n) {
       pos[i] = make_double4(0., 0., 0., -1.);
       vel[i] = make_double4(0., 0., 0., 0.);
-      angv[i] = make_double4(0., 0., 0., -1.);
+      angv[i] = make_double4(0., 0., 0., 0.);
       mat[i] = H4{};
       continue;
     }
@@ -240,7 +239,7 @@ int upload_particles(gdem_batch* b, const Slot& sl, const gdem_particles_desc* p
     const double* w = pd->angv ? pd->angv + 3 * i : nullptr;
     // ParticleSystem.cu:294-335: invm = 1/m, sqrtm = sqrt(m), moi = 2/5 m r^2
     vel[i] = make_double4(v ? v[0] : 0., v ? v[1] : 0., v ? v[2] : 0., 1. / m);
-    angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., (double)i);
+    angv[i] = make_double4(w ? w[0] : 0., w ? w[1] : 0., w ? w[2] : 0., 0.);
     const double moi = 2. / 5. * m * (r * r);
     mat[i].x = (H)pd->k[i];
     mat[i].y = (H)pd->etaconst[i];

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   L.nbr = B.nbr;
   L.nnb = B.nnb;
   L.ref = B.ref;
-  L.ref32 = B.ref32;
+  L.angv = B.angv[cur];
   L.wl = B.wl;
   L.nwl = B.nwl;
   L.tri = B.tri;
@@ -205,6 +205,12 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   for (int a = 0; a < 3; ++a) L.size[a] = S.size[a];
   L.cell_base = S.cell_base;
   L.skin = S.skin;
+  for (int a = 0; a < 3; ++a) {
+    L.qlo[a] = S.bmin[a];
+    L.qhi[a] = S.bmax[a];
+  }
+  double qeps;
+  refq_set(L.qlo, L.qhi, L.qh, qeps);
   L.nb_cap = B.nb_cap;
   L.wall_cap = B.wall_cap;
   L.ntri = S.ntri;
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
     A.wd = B.wd[q];
     A.nw = B.nw[q];
     A.ref = B.ref;
-    A.ref32 = B.ref32;
+    A.pid = B.pid[q];
     A.nbr = B.nbr;
     A.nnb = B.nnb;
     A.wl = B.wl;
@@ -268,6 +274,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
       A.bmax[a] = S.bmax[a];
     }
     A.threshSq = S.threshSq;
+    refq_set(A.bmin, A.bmax, A.qh, A.qeps);
   }
   const Own<H> own = load_own<false>(A, s, running);
   tile_p[threadIdx.x] = own.p;

@@ -564,7 +564,7 @@ struct StepArgs {
   Del3<H>* wd;
   int* nw;
   const double4* ref;
-  const float4* ref32;
+  const int* pid;  // original ids (read by the recorder only)
   const int2* nbr;
   const int* nnb;
   const int* wl;
@@ -575,6 +575,7 @@ struct StepArgs {
   double dt, mu;
   double g[3], bmin[3], bmax[3];
   double threshSq;
+  double qh[3], qeps;  // packed reference position: step per axis, error bound (refq_set)
 };
 
 // pair-list entries screened per batch
@@ -591,8 +592,7 @@ constexpr int SCREEN_BATCH = 8;
 // a loss for dilute ones.
 template <class H>
 struct Own {
-  double4 p, v, w;
-  float4 rf32;  // the reference position rounded to float (mixed form only)
+  double4 p, v, w;  // w.w: the packed reference position (refq_pack)
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
   int2 nb0[SCREEN_BATCH];
@@ -604,7 +604,6 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.p = make_double4(0., 0., 0., -1.);
   o.v = make_double4(0., 0., 0., 0.);
   o.w = o.v;
-  o.rf32 = make_float4(0.f, 0.f, 0.f, 0.f);
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
 #pragma unroll
@@ -614,7 +613,6 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     o.v = A.vin[s];
     o.w = A.win[s];
     o.mt = ld_nt(A.mat + s);
-    if constexpr (sizeof(H) != 8) o.rf32 = ld_nt(A.ref32 + s);
     o.nnb = ld_nt(A.nnb + s);
     o.nwl = ld_nt(A.nwl + s);
     o.nh0 = ld_nt(A.nh + s);
@@ -631,28 +629,69 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   return o;
 }
 
+// The position at the last list build travels packed in the unused fourth
+// word of the particle's spin record, which the step reads and writes anyway:
+// 21 bits per axis, a fixed-point offset from the box corner bmin in steps of
+// h = (bmax - bmin) / (2^21 - 1).  A position outside the box (never that of
+// an active particle) is stored as REFQ_EXACT instead.  The exact position
+// stays in its own array for the rare test that needs it.
+constexpr int REFQ_BITS = 21;
+constexpr unsigned long long REFQ_MASK = (1ull << REFQ_BITS) - 1;
+constexpr double REFQ_LEVELS = 2097151.0;  // 2^21 - 1
+constexpr unsigned long long REFQ_EXACT = ~0ull;
+
+// Step per axis and the bound on |ref - decoded ref|: each component is
+// rounded to the nearest step, so it is off by at most h/2, plus the
+// rounding of the double arithmetic that forms and decodes it (a few ulps
+// of |bmin| + |bmax|).
+__device__ __forceinline__ void refq_set(const double lo[3], const double hi[3], double h[3], double& eps) {
+  double hh = 0., big = 0.;
+  for (int a = 0; a < 3; ++a) {
+    h[a] = (hi[a] - lo[a]) * (1.0 / REFQ_LEVELS);
+    hh += h[a] * h[a];
+    big += fabs(lo[a]) + fabs(hi[a]);
+  }
+  eps = 0.5 * (1.0 + 1e-8) * sqrt(hh) + 4e-15 * big;
+}
+
+__device__ __forceinline__ double refq_pack(const double4& p, const double lo[3], const double hi[3],
+                                            const double h[3]) {
+  const double c[3] = {p.x, p.y, p.z};
+  unsigned long long u = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!(c[a] >= lo[a] && c[a] <= hi[a])) return __longlong_as_double((long long)REFQ_EXACT);
+    const double f = fmin(fmax(rint((c[a] - lo[a]) / h[a]), 0.), REFQ_LEVELS);
+    u |= (unsigned long long)f << (REFQ_BITS * a);
+  }
+  return __longlong_as_double((long long)u);
+}
+
 // The reference's refresh test (main.cu, cpu_dem.cu:1529-1552):
 // |x - ref|^2 > thresh in FP64, with ref the position at the last list
-// build.  It is first decided on ref rounded to float (16 B read per step
-// instead of 32): each rounded component is within half a float ulp,
-// |r| * 2^-24, of the exact one, so the distance from the rounded point is
-// within m = 2^-24 * (|r.x| + |r.y| + |r.z|) of the exact distance.  Outside
-// a band of twice that width around the threshold distance (plus a margin
-// for the double rounding of the distance itself) the two tests agree; in
-// it (rarely) the exact position is read and the reference's expression is
-// evaluated as is.
-__device__ __forceinline__ bool refresh_test(const double4& po, const float4 r32, const double4* ref,
-                                             double threshSq) {
-  const double ex = po.x - (double)r32.x, ey = po.y - (double)r32.y, ez = po.z - (double)r32.z;
-  const double d = sqrt(ex * ex + ey * ey + ez * ez);
-  const double t = sqrt(threshSq);
-  const double m = 2.0 * 5.9604644775390625e-8 * (fabs((double)r32.x) + fabs((double)r32.y) + fabs((double)r32.z)) +
-                   1e-12 * (d + t) + 1e-38;
-  if (d > t + m) return true;
-  if (d < t - m) return false;
+// build.  It is first decided on the packed position q: |ref - q| <= eps, so
+// the distance from q is within eps of the exact distance.  Outside a band of
+// twice that width around the threshold distance (plus a margin for the
+// double rounding of the distance itself) the two tests agree; in it
+// (rarely, a fraction of a micrometre for a box of half a metre) the exact
+// position is read and the reference's expression is evaluated as is.
+template <class H>
+__device__ __forceinline__ bool refresh_test(const double4& po, double tag, const double4* ref,
+                                             const StepArgs<H>& A) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(tag);
+  if (u != REFQ_EXACT) {
+    const double qx = fma((double)(unsigned)(u & REFQ_MASK), A.qh[0], A.bmin[0]);
+    const double qy = fma((double)(unsigned)((u >> REFQ_BITS) & REFQ_MASK), A.qh[1], A.bmin[1]);
+    const double qz = fma((double)(unsigned)((u >> (2 * REFQ_BITS)) & REFQ_MASK), A.qh[2], A.bmin[2]);
+    const double ex = po.x - qx, ey = po.y - qy, ez = po.z - qz;
+    const double d = sqrt(ex * ex + ey * ey + ez * ez);
+    const double t = sqrt(A.threshSq);
+    const double m = A.qeps + 1e-12 * (d + t) + 1e-38;
+    if (d > t + m) return true;
+    if (d < t - m) return false;
+  }
   const double4 rf = ld_nt(ref);
   const double fx = po.x - rf.x, fy = po.y - rf.y, fz = po.z - rf.z;
-  return (fx * fx + fy * fy + fz * fz) > threshSq;
+  return (fx * fx + fy * fy + fz * fz) > A.threshSq;
 }
 
 // ---- wave-level helpers (64 lanes) ------------------------------------------
@@ -787,7 +826,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const V3<T> wi{(T)w.x, (T)w.y, (T)w.z};
   V3<T> f{T(0), T(0), T(0)}, mom{T(0), T(0), T(0)};
   int nrec = 0;
-  const int self_id = (int)w.w;
+  int self_id = 0;
+  if constexpr (REC) self_id = A.pid[s];
 
   // The wave's wall candidates are spread over its lanes, 64 at a time; the
   // first 64 are fetched now so the load overlaps the pair pass.
@@ -1240,18 +1280,15 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   // reusing the values loaded at the start.  The mixed form keeps them (one
   // round trip fewer, measured faster).
   double4 p1, v1, w1;
-  float4 rf32;  // position at the last list build, rounded to float
   if constexpr (sizeof(T) == 8) {
     asm volatile("" ::: "memory");
     p1 = ld_nt(tile_p + tx);
     v1 = vin[s];
     w1 = win[s];
-    rf32 = ld_nt(A.ref32 + s);
   } else {
     p1 = p;
     v1 = v;
     w1 = w;
-    rf32 = own.rf32;
   }
   const double dt = A.dt;
   const double ax = (double)f.x * invm + A.g[0];
@@ -1287,7 +1324,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const bool out = po.x < A.bmin[0] || po.x > A.bmax[0] || po.y < A.bmin[1] || po.y > A.bmax[1] ||
                    po.z < A.bmin[2] || po.z > A.bmax[2];
   po.w = out ? -p1.w : p1.w;
-  if (!out) refresh = refresh_test(po, rf32, A.ref + s, A.threshSq);
+  if (!out) refresh = refresh_test(po, w1.w, A.ref + s, A);
   if (!(isfinite(po.x) && isfinite(po.y) && isfinite(po.z) && isfinite(vo.x) && isfinite(vo.y) &&
         isfinite(vo.z)))
     err |= GDEM_ERR_NAN_BIT;
@@ -1306,7 +1343,7 @@ struct ListArgs {
   int2* nbr;
   int* nnb;
   double4* ref;
-  float4* ref32;
+  double4* angv;  // spin records of the same state buffer as pos (fourth word: packed ref)
   int* wl;
   int* nwl;
   const TriGeo* tri;
@@ -1319,6 +1356,7 @@ struct ListArgs {
   int size[3];
   int cell_base;
   double skin;
+  double qlo[3], qhi[3], qh[3];  // box and step of the packed reference position
   int nb_cap, wall_cap;
   int ntri, wall_table;
 };
@@ -1464,7 +1502,7 @@ __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool acti
   {
     const double4 p = L.pos[s];
     L.ref[s] = p;
-    L.ref32[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+    L.angv[s].w = refq_pack(p, L.qlo, L.qhi, L.qh);
   }
   if (!active) {
     L.nnb[s] = 0;

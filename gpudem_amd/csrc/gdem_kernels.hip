@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   A.wd = B.wd[hc];
   A.nw = B.nw[hc];
   A.ref = B.ref;
-  A.ref32 = B.ref32;
+  A.pid = B.pid[hc];
   A.nbr = B.nbr;
   A.nnb = B.nnb;
   A.wl = B.wl;
@@ -56,6 +56,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
     A.bmax[a] = P.bmax[a];
   }
   A.threshSq = P.threshSq;
+  refq_set(A.bmin, A.bmax, A.qh, A.qeps);
   // this workgroup's positions, for in-tile neighbour reads
   __shared__ double4 tile_p[256];
   const bool valid = s < P.N;
@@ -388,7 +389,7 @@ __device__ __forceinline__ ListArgs list_args(const SimParams& P, const Bufs<H>&
   L.nbr = B.nbr;
   L.nnb = B.nnb;
   L.ref = B.ref;
-  L.ref32 = B.ref32;
+  L.angv = B.angv[cur];
   L.wl = B.wl;
   L.nwl = B.nwl;
   L.tri = B.tri;
@@ -401,6 +402,12 @@ __device__ __forceinline__ ListArgs list_args(const SimParams& P, const Bufs<H>&
   for (int a = 0; a < 3; ++a) L.size[a] = P.size[a];
   L.cell_base = 0;
   L.skin = P.skin;
+  for (int a = 0; a < 3; ++a) {
+    L.qlo[a] = P.bmin[a];
+    L.qhi[a] = P.bmax[a];
+  }
+  double qeps;
+  refq_set(L.qlo, L.qhi, L.qh, qeps);
   L.nb_cap = P.nb_cap;
   L.wall_cap = P.wall_cap;
   L.ntri = P.ntri;

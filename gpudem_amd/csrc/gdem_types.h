@@ -91,7 +91,7 @@ struct Bufs {
   using H4 = typename Vec4<H>::type;
   double4* pos[2];   // x, y, z, r   (r < 0 marks an inactive particle)
   double4* vel[2];   // vx, vy, vz, 1/m
-  double4* angv[2];  // wx, wy, wz, unused
+  double4* angv[2];  // wx, wy, wz, packed position at the last list build
   H4* mat[2];        // k, etaconst, sqrt(m), 1/I (FP64, or FP32 in the mixed form)
   int* pid[2];       // original particle id
   int* hkey[2];      // pair history keys, slot-major [hist_cap][N]
@@ -100,8 +100,7 @@ struct Bufs {
   int* wkey[2];      // wall history keys [whist_cap][N]
   Del3<H>* wd[2];    // [whist_cap][N]
   int* nw[2];
-  double4* ref;      // positions at the last list build
-  float4* ref32;     // the same rounded to float (the refresh test's first decision)
+  double4* ref;      // positions at the last list build (exact; the packed form is angv's w)
   int2* nbr;         // pair candidates (internal index, original id), slot-major [nb_cap][N]
   int* nnb;
   int* wl;           // wall candidates (triangle id) [wall_cap][N]
@@ -181,7 +180,6 @@ struct BBufs {
   Del3<H>* wd[2];
   int* nw[2];
   double4* ref;
-  float4* ref32;
   int2* nbr;
   int* nnb;
   int* wl;
