@@ -244,6 +244,51 @@ def test_inactive_particles_and_out_of_bounds(ctx):
     assert_state_equal(sc, orc, "inactive")
 
 
+GDEM_ERR_NB_OVERFLOW, GDEM_ERR_WALL_OVERFLOW, GDEM_ERR_HIST_OVERFLOW, GDEM_ERR_WHIST_OVERFLOW = 1, 2, 4, 8
+GDEM_E_SLOT_OVERFLOW = -4
+
+
+def corner_cluster():
+    """A 4x4x4 cubic lattice of r = 1 mm spheres, spacing 1.98 mm (every face
+    neighbour overlaps by 0.02 mm), in the corner of a 20 mm box: the lattice's
+    corner sphere overlaps the three walls there by 0.01 mm."""
+    r0, s = 1e-3, 1.98e-3
+    g = 0.99 * r0 + s * np.arange(4)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    verts = scenes.box_mesh((0., 0., 0.), (0.02, 0.02, 0.02))
+    bbox = scenes.bbox_rule(verts, [0., 0.02, 0., 0.02, 0., 0.02])
+    # skin 0.2 r: the face neighbours are candidates, the diagonal ones are not
+    params = dict(dt=1e-6, mu=0.5, g=(0., 0., -9.81), bbox=bbox, cell=2.2 * r0, skin=0.2 * r0, refresh=0.1 * r0)
+    return scenes.SceneSpec("corner_cluster", x, np.full(len(x), r0), verts, params, 2500., 1e7, 0.5)
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_capacity_overflow_is_reported(ctx, precision):
+    # Rows shorter than the packing needs: the list builders and the step keep
+    # to their rows and report the overflow through the status bits (the
+    # reference prints a warning, BoundingBox.cu:1146-1148, BVH.cu:526-528).
+    spec = corner_cluster()
+    m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
+    cases = [(dict(), 0),
+             (dict(nb_cap=2, hist_cap=1), GDEM_ERR_NB_OVERFLOW | GDEM_ERR_HIST_OVERFLOW),
+             (dict(wall_cap=1), GDEM_ERR_WALL_OVERFLOW),
+             (dict(whist_cap=1), GDEM_ERR_WHIST_OVERFLOW)]
+    for caps, bits in cases:
+        say(f"scene corner_cluster precision {precision} caps {caps}")
+        sc = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, dict(spec.params, precision=precision, **caps))
+        sc.step(5)
+        st = sc.status(check=False)
+        assert st["errors"] == bits, (caps, st)
+        assert st["rc"] == (GDEM_E_SLOT_OVERFLOW if bits else 0), (caps, st)
+        d = sc.download()
+        assert np.all(np.isfinite(d["x"])) and np.all(d["active"] == 1), caps
+        dbg = sc.debug()
+        assert np.all(dbg["nnb"] <= sc.caps["nb"]) and np.all(dbg["nwall"] <= sc.caps["wall"]), caps
+        assert np.all(dbg["nhist"] <= sc.caps["hist"]) and np.all(dbg["nwhist"] <= sc.caps["whist"]), caps
+        sc.close()
+
+
 # ---------------------------------------------------------------------------
 # mixed precision against FP64
 # ---------------------------------------------------------------------------
