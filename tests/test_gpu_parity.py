@@ -244,6 +244,51 @@ def test_inactive_particles_and_out_of_bounds(ctx):
     assert_state_equal(sc, orc, "inactive")
 
 
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_refresh_steps_match_oracle_near_threshold(ctx, precision):
+    # 64 spheres in free flight (no gravity, never within reach of each other
+    # or a wall) cross the refresh distance again and again.  Each step moves
+    # them 0.1-0.2 um, so the refresh test often falls within the error of the
+    # packed reference position (about 0.05 um on this 0.11 m box), where the
+    # exact position decides.  With no forces the mixed form integrates
+    # exactly like FP64, so both must rebuild on the oracle's steps.
+    rng = np.random.default_rng(21)
+    g = (np.arange(4) - 1.5) * 0.012
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    dirs = rng.normal(size=x.shape)
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    v = dirs * rng.uniform(0.01, 0.02, len(x))[:, None]
+    r0, dt, steps = 1e-3, 1e-5, 20000
+    verts = scenes.box_mesh((-0.05,) * 3, (0.05,) * 3)
+    bbox = scenes.bbox_rule(verts, [-0.05, 0.05] * 3)
+    params = dict(dt=dt, mu=0.5, g=(0., 0., 0.), bbox=bbox, cell=2.5 * r0, skin=0.5 * r0, refresh=0.25 * r0)
+    spec = scenes.SceneSpec("free_flight", x, np.full(len(x), r0), verts, params, 2500., 1e7, 0.5, v=v)
+    say(f"scene free_flight precision {precision} {steps} steps")
+    sc, orc = make_pair(ctx, spec, precision)
+    sc.step(steps)
+    orc.step(steps)
+    assert_state_equal(sc, orc, "free flight")
+    st = sc.status()
+    assert st["errors"] == 0 and st["rebuilds"] == orc.get("rebuilds"), (st, orc.get("rebuilds"))
+    # the same flight on the host: the oracle's rebuild rule, and how many
+    # tests fell within the packed position's error of the threshold
+    lo, hi = np.array(bbox[0::2]), np.array(bbox[1::2])
+    h = (hi - lo) * (1.0 / 2097151.0)
+    eps = 0.5 * np.sqrt(np.sum(h * h))
+    t = 0.25 * r0
+    p, ref, rebuilds, near = x.copy(), x.copy(), 0, 0
+    for _ in range(steps):
+        p = p + v * dt
+        q = lo + np.rint((ref - lo) / h) * h
+        near += int(np.sum(np.abs(np.linalg.norm(p - q, axis=1) - t) < eps))
+        if np.any(np.sum((p - ref) ** 2, axis=1) > t * t):
+            ref = p.copy()
+            rebuilds += 1
+    say(f"  rebuilds {st['rebuilds']} (host {rebuilds}), tests near the threshold {near}")
+    assert rebuilds == st["rebuilds"] >= 5 and near >= 5
+
+
 GDEM_ERR_NB_OVERFLOW, GDEM_ERR_WALL_OVERFLOW, GDEM_ERR_HIST_OVERFLOW, GDEM_ERR_WHIST_OVERFLOW = 1, 2, 4, 8
 GDEM_E_SLOT_OVERFLOW = -4
 
