@@ -75,6 +75,7 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
   }
   HIPCHK(c, A.alloc(&B.ref, N));
   HIPCHK(c, A.alloc(&B.nbr, N * sc->P.nb_cap));
+  HIPCHK(c, A.alloc(&B.inv, N));
   HIPCHK(c, A.alloc(&B.nnb, N));
   HIPCHK(c, A.alloc(&B.wl, N * sc->P.wall_cap));
   HIPCHK(c, A.alloc(&B.nwl, N));
@@ -699,7 +700,7 @@ int gdem_radix_sort(gdem_ctx* c, int n, const uint32_t* keys, uint32_t* keys_out
   for (int i = 0; i < n; ++i) iota[i] = i;
   HIPCHK(c, hipMemcpy(k0, keys, (size_t)n * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(i0, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-  launch_radix_sort(n, blocks, chunk, k0, i0, k1, i1, hist, ctl, false, c->stream);
+  launch_radix_sort(n, blocks, chunk, k0, i0, k1, i1, nullptr, hist, ctl, false, c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(keys_out, RS_FINAL ? k1 : k0, (size_t)n * 4, hipMemcpyDeviceToHost));

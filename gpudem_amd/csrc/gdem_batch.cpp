@@ -146,7 +146,7 @@ int64_t particle_bytes(int hbytes, int nb_cap, int wall_cap, int hist_cap, int w
          + 2 * (int64_t)hist_cap * (4 + h4) + 2 * 4   // pair histories
          + 2 * (int64_t)whist_cap * (4 + h4) + 2 * 4  // wall histories
          + 32                  // reference positions
-         + (int64_t)nb_cap * 8 + 4 + (int64_t)wall_cap * 4 + 4  // candidate lists
+         + (int64_t)nb_cap * 4 + 4 + (int64_t)wall_cap * 4 + 4  // candidate lists
          + 3 * 4;              // binning
 }
 constexpr int64_t kCellBytes = 4 + 8 + 4;    // counter, (first, end), wall-table offset

@@ -142,9 +142,14 @@ __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
   SlotCtl& C = B.sctl[sid];
   const int q = __builtin_amdgcn_readfirstlane(C.q);
   const int cur = (step & 1) ^ q;
+  // new slot of slot j of this scene
+  auto dest = [&](int j) {
+    const int c = B.bin_cell[j];
+    const int r = B.bin_rank[j];
+    return c >= 0 ? B.cell_tab[c].x + r : S.seg0 + C.nact + r;
+  };
   const int c = B.bin_cell[i];
-  const int r = B.bin_rank[i];
-  const int d = c >= 0 ? B.cell_tab[c].x + r : S.seg0 + C.nact + r;
+  const int d = dest(i);
   if (d < S.seg0 || d >= S.seg0 + S.nseg) {
     atomicOr(&C.err, GDEM_ERR_NB_OVERFLOW_BIT);  // inconsistent counts: never write outside the segment
     return;
@@ -159,7 +164,8 @@ __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
   const int nh = B.nh[q][i];
   B.nh[q ^ 1][d] = nh;
   for (int k = 0; k < nh; ++k) {
-    B.hkey[q ^ 1][k * N + d] = B.hkey[q][k * N + i];
+    const int j = B.hkey[q][k * N + i];  // the partner's old slot, in this scene
+    B.hkey[q ^ 1][k * N + d] = (j >= S.seg0 && j < S.seg0 + S.nseg) ? dest(j) : -1;
     B.hd[q ^ 1][k * N + d] = B.hd[q][k * N + i];
   }
   const int nw = B.nw[q][i];

@@ -489,28 +489,29 @@ constexpr int step_waves() {
   return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
 }
 
-// Touching pairs of a particle whose list entries the screening pass keeps in
-// LDS for the force pass (later ones are read again from the list).
-constexpr int HIT_STASH = 8;
+// Touching pairs of a particle whose list entries (partner slots) the
+// screening pass keeps in LDS for the force pass (later ones are read again
+// from the list).
+constexpr int HIT_STASH = 16;
 
 // The hit stash is used only in the pair pass and the vertex/edge features
 // only in the wall pass, so they share their LDS: a thread's VE_MAX feature
-// words are the two halves of its own HIT_STASH stash entries (ve_slot), so no
-// thread's features overlap another thread's stash, whatever pass the other
-// waves of the workgroup are in.
-constexpr int VE_MAX = 2 * HIT_STASH;
+// words are its own HIT_STASH stash words (ve_slot), so no thread's features
+// overlap another thread's stash, whatever pass the other waves of the
+// workgroup are in.
+constexpr int VE_MAX = HIT_STASH;
 
 template <int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
-  int2 hit[HIT_STASH][256];
+  int hit[HIT_STASH][256];
 };
 
 // feature word u (< VE_MAX) of thread tx
 template <class Lds>
 __device__ __forceinline__ int* ve_slot(Lds& lds, int tx, int u) {
-  return reinterpret_cast<int*>(&lds.hit[u >> 1][tx]) + (u & 1);
+  return &lds.hit[u][tx];
 }
 
 template <class Row>
@@ -565,7 +566,7 @@ struct StepArgs {
   int* nw;
   const double4* ref;
   const int* pid;  // original ids (read by the recorder only)
-  const int2* nbr;
+  const int* nbr;
   const int* nnb;
   const int* wl;
   const int* nwl;
@@ -595,7 +596,7 @@ struct Own {
   double4 p, v, w;  // w.w: the packed reference position (refq_pack)
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
-  int2 nb0[SCREEN_BATCH];
+  int nb0[SCREEN_BATCH];
 };
 
 template <bool PRE, class H>
@@ -607,7 +608,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
 #pragma unroll
-  for (int u = 0; u < SCREEN_BATCH; ++u) o.nb0[u] = make_int2(-1, 0);
+  for (int u = 0; u < SCREEN_BATCH; ++u) o.nb0[u] = -1;
   if (valid) {
     o.p = A.pin[s];
     o.v = A.vin[s];
@@ -812,10 +813,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // wait for the last entry would come after the wall-list and candidate
     // position loads below, and would wait for those too.
     static_assert(SCREEN_BATCH == 8, "the wait below names eight entries");
-    asm volatile("" ::"v"(own.nb0[0].x), "v"(own.nb0[0].y), "v"(own.nb0[1].x), "v"(own.nb0[1].y),
-                 "v"(own.nb0[2].x), "v"(own.nb0[2].y), "v"(own.nb0[3].x), "v"(own.nb0[3].y));
-    asm volatile("" ::"v"(own.nb0[4].x), "v"(own.nb0[4].y), "v"(own.nb0[5].x), "v"(own.nb0[5].y),
-                 "v"(own.nb0[6].x), "v"(own.nb0[6].y), "v"(own.nb0[7].x), "v"(own.nb0[7].y));
+    asm volatile("" ::"v"(own.nb0[0]), "v"(own.nb0[1]), "v"(own.nb0[2]), "v"(own.nb0[3]), "v"(own.nb0[4]),
+                 "v"(own.nb0[5]), "v"(own.nb0[6]), "v"(own.nb0[7]));
   }
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
   const double invmoi = (double)mt.w;
@@ -845,14 +844,13 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   // records; their positions are requested now, before the history keys
   // below, because any LDS access after those loads (these tile reads
   // included) waits for them.
-  int2 ev0[SCREEN_BATCH];
+  int ev0[SCREEN_BATCH];
   double4 cv0[SCREEN_BATCH];
   if constexpr (PRE) {
 #pragma unroll
-    for (int u = 0; u < SCREEN_BATCH; ++u) ev0[u] = u < nnb ? own.nb0[u] : make_int2(-1, 0);
+    for (int u = 0; u < SCREEN_BATCH; ++u) ev0[u] = u < nnb ? own.nb0[u] : -1;
 #pragma unroll
-    for (int u = 0; u < SCREEN_BATCH; ++u)
-      cv0[u] = ev0[u].x >= 0 ? pos_of(ev0[u].x) : make_double4(0., 0., 0., -1.);
+    for (int u = 0; u < SCREEN_BATCH; ++u) cv0[u] = ev0[u] >= 0 ? pos_of(ev0[u]) : make_double4(0., 0., 0., -1.);
   }
 
   // history rows of pairs and walls
@@ -881,8 +879,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // The first HIT_STASH touching entries go to this thread's LDS column.
     uint64_t hits = 0;
     int nhit = 0;
-    int2* stash = &lds.hit[0][tx];
-    auto test = [&](const int q0, const int2* ev, const double4* cv) {
+    int* stash = &lds.hit[0][tx];
+    auto test = [&](const int q0, const int* ev, const double4* cv) {
 #pragma unroll
       for (int u = 0; u < SCREEN_BATCH; ++u) {
         const double4 pj = cv[u];
@@ -904,14 +902,12 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       q_start = SCREEN_BATCH;
     }
     for (int q0 = q_start; q0 < nnb; q0 += SCREEN_BATCH) {
-      int2 ev[SCREEN_BATCH];
+      int ev[SCREEN_BATCH];
       double4 cv[SCREEN_BATCH];
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u)
-        ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : make_int2(-1, 0);
+      for (int u = 0; u < SCREEN_BATCH; ++u) ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : -1;
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u)
-        cv[u] = ev[u].x >= 0 ? pos_of(ev[u].x) : make_double4(0., 0., 0., -1.);
+      for (int u = 0; u < SCREEN_BATCH; ++u) cv[u] = ev[u] >= 0 ? pos_of(ev[u]) : make_double4(0., 0., 0., -1.);
       test(q0, ev, cv);
     }
     // the entries are read from other lanes' columns below
@@ -935,7 +931,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // cannot count the reads in flight and waits for all of them, the
     // previous round's history stores included.
     struct PairIn {
-      int2 e;       // (partner slot, partner id)
+      int e;        // partner slot (also the history key)
       int slot;     // history slot, -1 for a new contact
       int so;       // owner's slot
       int o, o_first, o_n;  // owner's lane, its first pair's index, its history length
@@ -959,23 +955,23 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o_n = __shfl(nh0, o);
       const int r = g - o_first;  // rank among the owner's touching pairs
       // (the LDS read is marked non-temporal so it stays an LDS instruction)
-      int2 e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
+      int e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
       if (has && r >= HIT_STASH) {
         e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
         // wait for it here, inside the rare branch (see above)
-        asm volatile("" ::"v"(e.x), "v"(e.y));
+        asm volatile("" ::"v"(e));
       }
       q.so = so;
       q.o = o;
       q.o_first = o_first;
       q.o_n = o_n;
-      q.e = has ? e : make_int2(so, -1);
-      q.pj = pin[q.e.x];
-      q.vj = vin[q.e.x];
-      q.wj = win[q.e.x];
+      q.e = has ? e : so;
+      q.pj = pin[q.e];
+      q.vj = vin[q.e];
+      q.wj = win[q.e];
       // (keeps the compiler from sinking the loads above below the search)
       asm volatile("" ::: "memory");
-      q.slot = has ? row.find_in(wtx + o, so, o_n, e.y) : -1;
+      q.slot = has ? row.find_in(wtx + o, so, o_n, e) : -1;
       q.d = ld_nt(row.gd + (size_t)(q.slot >= 0 ? q.slot : 0) * N + q.so);
     };
     auto pair_finish = [&](const int base, const PairIn& q) {
@@ -993,7 +989,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
       const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
       const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
-      const int2 e = q.e;
+      const int e = q.e;
       const int slot = q.slot;
       const double4 pj = q.pj, vj = q.vj, wj = q.wj;
       // new contacts take their particle's next free slots in list order
@@ -1006,7 +1002,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           const int ns = o_n + o_added + __popcll(fresh & before_me);
           if (ns < row.cap) {
             wslot = ns;
-            st_nt(e.y, row.gkey + (size_t)ns * N + so);
+            st_nt(e, row.gkey + (size_t)ns * N + so);
           } else {
             err |= GDEM_ERR_HIST_OVERFLOW_BIT;
           }
@@ -1061,7 +1057,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         int rkey = 0;
         double rdel = 0.;
         if constexpr (REC) {
-          rkey = __shfl(e.y, src);
+          rkey = __shfl(e, src);
           rdel = __shfl(delMagD, src);
         }
         if (mine) {
@@ -1078,7 +1074,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           if constexpr (REC) {
             if (nrec < rec.cap) {
               const size_t r = (size_t)self_id * rec.cap + nrec;
-              rec.key[r] = rkey;
+              rec.key[r] = A.pid[rkey];  // the partner's original id
               rec.kind[r] = 0;
               rec.fn[r * 3] = a0; rec.fn[r * 3 + 1] = a1; rec.fn[r * 3 + 2] = a2;
               rec.ft[r * 3] = b0; rec.ft[r * 3 + 1] = b1; rec.ft[r * 3 + 2] = b2;
@@ -1340,7 +1336,7 @@ struct ListArgs {
   const int* pid;
   const int* cell_of;
   const int2* cell_tab;
-  int2* nbr;
+  int* nbr;
   int* nnb;
   double4* ref;
   double4* angv;  // spin records of the same state buffer as pos (fourth word: packed ref)
@@ -1464,7 +1460,7 @@ __device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col
     }
     col[(j + 1) * PL_BLOCK] = x;
   }
-  for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * PL_BLOCK];
+  for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * PL_BLOCK].x;
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
 }
@@ -1492,7 +1488,7 @@ __device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
     cand[at] = q;
     ++n;
   });
-  for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = make_int2(cand[q], cpid[q]);
+  for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = cand[q];
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
 }

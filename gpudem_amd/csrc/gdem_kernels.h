@@ -94,7 +94,7 @@ void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc
 template <class H>
 void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st);
 
-void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1,
+void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1, int* inv_out,
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st);
 void launch_tri_probe(const TriGeo* tri, int np, const double* pts, const double* rsq, double* out,
                       int* hit, hipStream_t st);

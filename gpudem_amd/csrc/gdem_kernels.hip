@@ -218,9 +218,12 @@ __device__ __forceinline__ void rs_clear_wcnt(uint16_t* wcnt) {
   __syncthreads();
 }
 
+// inv (the last pass of a rebuild's sort, else null): inv[source index] =
+// its sorted position, the inverse of the permutation in iout
 __device__ __forceinline__ void rs_scatter_block(int n, int chunk, int pass, const uint32_t* __restrict__ kin,
                                                  const int* __restrict__ iin, uint32_t* __restrict__ kout,
-                                                 int* __restrict__ iout, uint32_t* base, uint16_t* wcnt, int b) {
+                                                 int* __restrict__ iout, int* __restrict__ inv, uint32_t* base,
+                                                 uint16_t* wcnt, int b) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
@@ -268,6 +271,7 @@ __device__ __forceinline__ void rs_scatter_block(int n, int chunk, int pass, con
       if (pos < (uint32_t)n) {  // always, unless a grid barrier failed
         kout[pos] = k;
         iout[pos] = idx;
+        if (inv && (unsigned)idx < (unsigned)n) inv[idx] = (int)pos;
       }
       if (rank == 0 && last_wave) base[d] = bd + before + cw;
     }
@@ -291,7 +295,8 @@ __device__ __forceinline__ void permute_one(const SimParams& P, const Bufs<H>& B
   const int nh = B.nh[hc][src];
   B.nh[hc ^ 1][s] = nh;
   for (int q = 0; q < nh; ++q) {
-    B.hkey[hc ^ 1][(size_t)q * N + s] = B.hkey[hc][(size_t)q * N + src];
+    const int k = B.hkey[hc][(size_t)q * N + src];  // the partner's old slot
+    B.hkey[hc ^ 1][(size_t)q * N + s] = (unsigned)k < (unsigned)N ? B.inv[k] : -1;
     B.hd[hc ^ 1][(size_t)q * N + s] = B.hd[hc][(size_t)q * N + src];
   }
   const int nw = B.nw[hc][src];
@@ -353,6 +358,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int
                                                            const int* __restrict__ iin,
                                                            uint32_t* __restrict__ kout,
                                                            int* __restrict__ iout,
+                                                           int* __restrict__ inv,
                                                            const uint32_t* __restrict__ hist,
                                                            const int* ctl) {
   if (GATED && !rebuild_due(ctl)) return;
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int n, int chunk, int
   __shared__ uint16_t wcnt[RS_WAVES * RS_DIGITS];
   const int nb = gridDim.x;
   rs_bases(hist, hist + (size_t)RS_DIGITS * nb, blockIdx.x, nb, base);
-  rs_scatter_block(n, chunk, pass, kin, iin, kout, iout, base, wcnt, blockIdx.x);
+  rs_scatter_block(n, chunk, pass, kin, iin, kout, iout, inv, base, wcnt, blockIdx.x);
 }
 
 template <class H>
@@ -542,7 +548,8 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
     rs_scan_digits(hist, dtot, G, b, G);
     if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
     rs_bases(hist, dtot, b, G, dig);
-    rs_scatter_block(N, P.rs_chunk, pass, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1], dig, wcnt, b);
+    rs_scatter_block(N, P.rs_chunk, pass, B.key[a], B.idx[a], B.key[a ^ 1], B.idx[a ^ 1],
+                     pass == RS_PASSES - 1 ? B.inv : nullptr, dig, wcnt, b);
     if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
   }
   // The cell table needs the sorted keys (complete at the last barrier) and
@@ -597,14 +604,15 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
   D.morton[id] = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
   const int nn = B.nnb[s];
   D.nnb[id] = nn;
-  for (int q = 0; q < nn; ++q) D.nb[(size_t)id * P.nb_cap + q] = B.nbr[(size_t)q * N + s].y;
+  for (int q = 0; q < nn; ++q) D.nb[(size_t)id * P.nb_cap + q] = B.pid[hc][B.nbr[(size_t)q * N + s]];
   const int nw = B.nwl[s];
   D.nwall[id] = nw;
   for (int q = 0; q < nw; ++q) D.wall[(size_t)id * P.wall_cap + q] = B.wl[(size_t)q * N + s];
   const int nh = B.nh[hc][s];
   D.nhist[id] = nh;
   for (int q = 0; q < nh; ++q) {
-    D.hist_key[(size_t)id * P.hist_cap + q] = B.hkey[hc][(size_t)q * N + s];
+    const int k = B.hkey[hc][(size_t)q * N + s];  // the partner's slot
+    D.hist_key[(size_t)id * P.hist_cap + q] = (unsigned)k < (unsigned)N ? B.pid[hc][k] : -1;
     const auto d = B.hd[hc][(size_t)q * N + s];
     D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + 0] = (double)d.x;
     D.hist_delta[((size_t)id * P.hist_cap + q) * 3 + 1] = (double)d.y;
@@ -621,8 +629,18 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
   }
 }
 
+// inv[original id] = current slot, for k_set_history (inv is free between
+// rebuilds)
+template <class H>
+__global__ __launch_bounds__(256) void k_slot_of_id(SimParams P, Bufs<H> B) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.N) return;
+  const int id = B.pid[B.ctl[CTL_HCUR]][s];
+  if ((unsigned)id < (unsigned)P.N) B.inv[id] = s;
+}
+
 // Contact histories from original particle order into the current internal
-// order (snapshot restore).  Rows longer than the capacity raise the
+// order (snapshot restore); pair keys become partner slots.  Rows longer than the capacity raise the
 // overflow bits and are cut at the capacity.
 template <class H>
 __global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, HistIn in) {
@@ -640,7 +658,8 @@ __global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, His
   B.nh[hc][s] = nh;
   for (int q = 0; q < nh; ++q) {
     const size_t src = (size_t)id * P.hist_cap + q;
-    B.hkey[hc][(size_t)q * N + s] = in.hist_key[src];
+    const int k = in.hist_key[src];  // the partner's original id -> its slot
+    B.hkey[hc][(size_t)q * N + s] = (unsigned)k < (unsigned)N ? B.inv[k] : -1;
     Del3<H> d;
     d.x = (H)in.hist_delta[src * 3];
     d.y = (H)in.hist_delta[src * 3 + 1];
@@ -746,20 +765,21 @@ void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, h
   k_step<T, H, true, HistScan<H>, HistScan<H>><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
 }
 
-void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1,
+void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1, int* inv_out,
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st) {
   uint32_t* kk[2] = {k0, k1};
   int* ii[2] = {i0, i1};
   for (int pass = 0; pass < RS_PASSES; ++pass) {
     const int a = pass & 1, b = a ^ 1;
+    int* inv = pass == RS_PASSES - 1 ? inv_out : nullptr;
     if (gated) {
       k_rs_count<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], hist, ctl);
       k_rs_scan<true><<<RS_SCAN_BLOCKS, RS_THREADS, 0, st>>>(blocks, hist, ctl);
-      k_rs_scatter<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
+      k_rs_scatter<true><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], inv, hist, ctl);
     } else {
       k_rs_count<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], hist, ctl);
       k_rs_scan<false><<<RS_SCAN_BLOCKS, RS_THREADS, 0, st>>>(blocks, hist, ctl);
-      k_rs_scatter<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], hist, ctl);
+      k_rs_scatter<false><<<blocks, RS_THREADS, 0, st>>>(n, chunk, pass, kk[a], ii[a], kk[b], ii[b], inv, hist, ctl);
     }
   }
 }
@@ -795,7 +815,7 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
   }
   const int g = blocks_for(P.N, 256);
   k_keys<H><<<g, 256, 0, st>>>(P, B);
-  launch_radix_sort(P.N, P.rs_blocks, P.rs_chunk, B.key[0], B.idx[0], B.key[1], B.idx[1], B.rs_hist,
+  launch_radix_sort(P.N, P.rs_blocks, P.rs_chunk, B.key[0], B.idx[0], B.key[1], B.idx[1], B.inv, B.rs_hist,
                     B.ctl, true, st);
   k_permute<H><<<g, 256, 0, st>>>(P, B);
   k_cells<H><<<g, 256, 0, st>>>(P, B);
@@ -821,6 +841,7 @@ void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc
 
 template <class H>
 void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st) {
+  k_slot_of_id<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B);
   k_set_history<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, in);
 }
 

@@ -94,15 +94,16 @@ struct Bufs {
   double4* angv[2];  // wx, wy, wz, packed position at the last list build
   H4* mat[2];        // k, etaconst, sqrt(m), 1/I (FP64, or FP32 in the mixed form)
   int* pid[2];       // original particle id
-  int* hkey[2];      // pair history keys, slot-major [hist_cap][N]
+  int* hkey[2];      // pair history keys (partner slot), slot-major [hist_cap][N]
   Del3<H>* hd[2];    // pair history deltas, slot-major [hist_cap][N]
   int* nh[2];
   int* wkey[2];      // wall history keys [whist_cap][N]
   Del3<H>* wd[2];    // [whist_cap][N]
   int* nw[2];
   double4* ref;      // positions at the last list build (exact; the packed form is angv's w)
-  int2* nbr;         // pair candidates (internal index, original id), slot-major [nb_cap][N]
+  int* nbr;          // pair candidates (partner slot), slot-major [nb_cap][N], ascending original id
   int* nnb;
+  int* inv;          // new slot of each old slot at a rebuild (history keys are remapped through it)
   int* wl;           // wall candidates (triangle id) [wall_cap][N]
   int* nwl;
   uint32_t* key[2];  // radix sort ping-pong
@@ -180,7 +181,7 @@ struct BBufs {
   Del3<H>* wd[2];
   int* nw[2];
   double4* ref;
-  int2* nbr;
+  int* nbr;
   int* nnb;
   int* wl;
   int* nwl;

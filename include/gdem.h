@@ -122,7 +122,7 @@ typedef struct {
   int* nnb;             /* [n] */
   int* wall;            /* [n*wall_cap] wall candidates (triangle ids), ascending */
   int* nwall;           /* [n] */
-  int* hist_key;        /* [n*hist_cap] pair history keys in slot order */
+  int* hist_key;        /* [n*hist_cap] pair history keys (partner original ids) in slot order */
   double* hist_delta;   /* [n*hist_cap*3] */
   int* nhist;           /* [n] */
   int* whist_key;       /* [n*whist_cap] */
