@@ -579,24 +579,25 @@ struct StepArgs {
   double qh[3], qeps;  // packed reference position: step per axis, error bound (refq_set)
 };
 
-// pair-list entries screened per batch
+// pair-list entries screened per batch, and (PRE) fetched with the own records
 constexpr int SCREEN_BATCH = 8;
+constexpr int PRE_ROWS = 2 * SCREEN_BATCH;
 
 // A particle's own records: state, material, list and history lengths, and
-// (PRE) the first SCREEN_BATCH pair-list entries.  All loads are
-// independent, so they share one memory round trip; lanes without a particle
-// load nothing.  The entries are fetched before the list length is known:
-// rows past the length (always inside the [nb_cap][N] array) hold stale
-// entries, which the screening pass masks.  This saves the round trip that
-// waits for the length, at the price of reading SCREEN_BATCH rows whatever
-// the length: a gain for dense packings (lists of about eight and longer),
-// a loss for dilute ones.
+// (PRE) the first PRE_ROWS pair-list entries (two screening batches).  All
+// loads are independent, so they share one memory round trip; lanes without
+// a particle load nothing.  The entries are fetched before the list length is
+// known: rows past the length (always inside the [nb_cap][N] array) hold
+// stale entries, which the screening pass masks.  This saves the round trips
+// that wait for the length and for the second batch's entries, at the price
+// of reading PRE_ROWS 4-byte rows whatever the length: a gain for dense
+// packings (lists of about eight and longer), a loss for dilute ones.
 template <class H>
 struct Own {
   double4 p, v, w;  // w.w: the packed reference position (refq_pack)
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
-  int nb0[SCREEN_BATCH];
+  int nb0[PRE_ROWS];
 };
 
 template <bool PRE, class H>
@@ -608,7 +609,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
 #pragma unroll
-  for (int u = 0; u < SCREEN_BATCH; ++u) o.nb0[u] = -1;
+  for (int u = 0; u < PRE_ROWS; ++u) o.nb0[u] = -1;
   if (valid) {
     o.p = A.pin[s];
     o.v = A.vin[s];
@@ -623,7 +624,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     // for all of them before the next dependent access)
     if constexpr (PRE) {
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u)
+      for (int u = 0; u < PRE_ROWS; ++u)
         o.nb0[u] = A.nbr[(size_t)(u < A.nb_cap ? u : A.nb_cap - 1) * A.N + s];
     }
   }
@@ -812,9 +813,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // waited for here (the empty asm uses them): left to the compiler, the
     // wait for the last entry would come after the wall-list and candidate
     // position loads below, and would wait for those too.
-    static_assert(SCREEN_BATCH == 8, "the wait below names eight entries");
+    static_assert(PRE_ROWS == 16, "the wait below names sixteen entries");
     asm volatile("" ::"v"(own.nb0[0]), "v"(own.nb0[1]), "v"(own.nb0[2]), "v"(own.nb0[3]), "v"(own.nb0[4]),
                  "v"(own.nb0[5]), "v"(own.nb0[6]), "v"(own.nb0[7]));
+    asm volatile("" ::"v"(own.nb0[8]), "v"(own.nb0[9]), "v"(own.nb0[10]), "v"(own.nb0[11]), "v"(own.nb0[12]),
+                 "v"(own.nb0[13]), "v"(own.nb0[14]), "v"(own.nb0[15]));
   }
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
   const double invmoi = (double)mt.w;
@@ -897,9 +900,17 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     };
     int q_start = 0;
     if constexpr (PRE) {
-      // (outside the loop, so that no wait for a later batch's loads precedes it)
+      // (outside the loop, so that no wait for a later batch's loads precedes
+      // it; likewise the second batch, whose entries are in registers)
       test(0, ev0, cv0);
-      q_start = SCREEN_BATCH;
+      int ev1[SCREEN_BATCH];
+      double4 cv1[SCREEN_BATCH];
+#pragma unroll
+      for (int u = 0; u < SCREEN_BATCH; ++u) ev1[u] = SCREEN_BATCH + u < nnb ? own.nb0[SCREEN_BATCH + u] : -1;
+#pragma unroll
+      for (int u = 0; u < SCREEN_BATCH; ++u) cv1[u] = ev1[u] >= 0 ? pos_of(ev1[u]) : make_double4(0., 0., 0., -1.);
+      test(SCREEN_BATCH, ev1, cv1);
+      q_start = PRE_ROWS;
     }
     for (int q0 = q_start; q0 < nnb; q0 += SCREEN_BATCH) {
       int ev[SCREEN_BATCH];
