@@ -1378,7 +1378,7 @@ constexpr int NB_MAX = 64;
 // time with their positions and ids loaded together (a few memory round trips
 // per slab instead of one per cell and per particle).  visit(q, id) is called
 // for every candidate in range.
-constexpr int PL_BATCH = 8;
+constexpr int PL_BATCH = 16;
 
 template <class Visit>
 __device__ __forceinline__ void pair_candidates(const ListArgs& L, int s, const double4& p, Visit&& visit) {
