@@ -41,9 +41,15 @@ struct gdem_scene {
   hipEvent_t t0 = nullptr, t1 = nullptr;
   double last_total_ms = 0, last_force_ms = 0;
   int last_force_launches = 0;
+  int chain_blocks = 1;               // radix-sort grid of the launch chain
+  uint32_t* h_abort = nullptr;        // pinned copy of the barrier's abort word
+  hipEvent_t abort_ev = nullptr;      // recorded after that copy
+  bool abort_probe = false;           // a copy is in flight or done
 
   ~gdem_scene() {
     if (graph) (void)hipGraphExecDestroy(graph);
+    if (abort_ev) (void)hipEventDestroy(abort_ev);
+    if (h_abort) (void)hipHostFree(h_abort);
     for (auto e : ev) (void)hipEventDestroy(e);
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
@@ -82,7 +88,7 @@ int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
   HIPCHK(c, A.alloc(&B.cell_tmp, N));
   HIPCHK(c, A.alloc(&B.cell_of, N));
   HIPCHK(c, A.alloc(&B.cell_tab, ncell));
-  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)RS_DIGITS * sc->P.rs_blocks + RS_DIGITS));
+  HIPCHK(c, A.alloc(&B.rs_hist, (size_t)RS_DIGITS * std::max(sc->P.rs_blocks, sc->chain_blocks) + RS_DIGITS));
   HIPCHK(c, A.alloc(&B.gbar, GBAR_WORDS));
   HIPCHK(c, hipMemsetAsync(B.gbar, 0, GBAR_WORDS * sizeof(uint32_t), c->stream));
   HIPCHK(c, hipMemsetAsync(B.cell_tab, 0xff, ncell * sizeof(int2), c->stream));
@@ -118,11 +124,65 @@ int enqueue_steps(gdem_scene* sc, int n) {
   return GDEM_OK;
 }
 
+// A grid barrier of the rebuild kernel was cancelled (GDEM_ERR_BARRIER): the
+// kernel stopped before any phase after it, so the state, the lists and the
+// cell of every slot are those of the last complete rebuild and the refresh
+// flag is still set.  The scene switches to the launch chain, which needs no
+// co-resident grid, and rebuilds through it at once.  `wait`: synchronise
+// and read the abort word; otherwise use the copy the last gdem_scene_step
+// left, if it has landed.
+template <class H>
+int recover_barrier(gdem_scene* sc, bool wait) {
+  if (!sc->coop) return GDEM_OK;
+  gdem_ctx* c = sc->ctx;
+  hipStream_t st = c->stream;
+  Bufs<H>& B = bufs<H>(sc);
+  uint32_t ab = 0;
+  if (wait) {
+    HIPCHK(c, hipStreamSynchronize(st));
+    HIPCHK(c, hipMemcpy(&ab, B.gbar + GBAR_ABORT, 4, hipMemcpyDeviceToHost));
+  } else {
+    if (!sc->abort_probe || hipEventQuery(sc->abort_ev) != hipSuccess) return GDEM_OK;
+    ab = *sc->h_abort;
+  }
+  if (!ab) return GDEM_OK;
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipMemsetAsync(B.gbar, 0, GBAR_WORDS * sizeof(uint32_t), st));
+  sc->coop = false;
+  sc->abort_probe = false;
+  SimParams& P = sc->P;
+  P.rs_blocks = sc->chain_blocks;
+  P.rs_chunk = ((P.N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
+  if (sc->graph) {
+    HIPCHK(c, hipGraphExecDestroy(sc->graph));
+    sc->graph = nullptr;
+  }
+  HIPCHK(c, launch_rebuild<H>(P, B, false, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  return GDEM_OK;
+}
+
+// After the steps of a call, the abort word is copied to pinned host memory
+// behind them, so the next call can see a cancelled barrier without waiting.
+template <class H>
+int probe_barrier(gdem_scene* sc) {
+  if (!sc->coop) return GDEM_OK;
+  gdem_ctx* c = sc->ctx;
+  HIPCHK(c, hipMemcpyAsync(sc->h_abort, bufs<H>(sc).gbar + GBAR_ABORT, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(sc->abort_ev, c->stream));
+  sc->abort_probe = true;
+  return GDEM_OK;
+}
+
 template <class H>
 int step_impl(gdem_scene* sc, int nsteps) {
   gdem_ctx* c = sc->ctx;
   hipStream_t st = c->stream;
   Bufs<H>& B = bufs<H>(sc);
+  {
+    const int rc = recover_barrier<H>(sc, false);
+    if (rc) return rc;
+  }
   if (sc->timing > 0) {
     // direct launches; an event pair brackets the force kernel of every
     // `timing`-th step (sampling keeps the host ahead of the device)
@@ -154,7 +214,7 @@ int step_impl(gdem_scene* sc, int nsteps) {
     sc->last_force_ms = fsum;
     sc->last_force_launches = nsamp;
     sc->steps += nsteps;
-    return GDEM_OK;
+    return probe_barrier<H>(sc);
   }
   int left = nsteps;
   if (sc->graph_steps > 0 && nsteps >= sc->graph_steps) {
@@ -177,7 +237,7 @@ int step_impl(gdem_scene* sc, int nsteps) {
   if (erc) return erc;
   HIPCHK(c, hipGetLastError());
   sc->steps += nsteps;
-  return GDEM_OK;
+  return probe_barrier<H>(sc);
 }
 
 template <class H>
@@ -197,8 +257,8 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   // and the rebuild phases are memory-bound loops that do not need more.
   const int limit = sc->chain ? 0 : std::min(1024, coop_grid_limit<H>() / 2);
   sc->coop = limit > 0;
-  P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want))
-                         : std::max(1, std::min(1024, (N + 2047) / 2048));
+  sc->chain_blocks = std::max(1, std::min(1024, (N + 2047) / 2048));
+  P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want)) : sc->chain_blocks;
   P.rs_chunk = ((N + P.rs_blocks - 1) / P.rs_blocks + 255) / 256 * 256;
   P.list_blocks = list_grid<H>(N);
   // GDEM_GBAR_SPIN (environment) lowers the barrier's poll limit, to exercise
@@ -240,6 +300,8 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   HIPCHK(c, hipMemcpyAsync(sc->d_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
   HIPCHK(c, launch_rebuild<H>(sc->P, B, sc->coop, st));
   HIPCHK(c, hipStreamSynchronize(st));
+  rc = recover_barrier<H>(sc, true);
+  if (rc) return rc;
   int rb = 0;
   HIPCHK(c, hipMemcpy(&rb, sc->d_ctl + CTL_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
   sc->rebuild_base = rb;
@@ -314,6 +376,35 @@ int debug_impl(gdem_scene* sc, gdem_debug_view* vw) {
   rc |= cp(vw->whist_key, D.whist_key, N * P.whist_cap * 4);
   rc |= cp(vw->whist_delta, D.whist_delta, N * P.whist_cap * 24);
   rc |= cp(vw->nwhist, D.nwhist, N * 4);
+  return rc ? GDEM_E_HIP : GDEM_OK;
+}
+
+template <class H>
+int debug_order_impl(gdem_scene* sc, gdem_order_view* vw) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  DevArena tmp;
+  OrderDev D{};
+  HIPCHK(c, tmp.alloc(&D.key, N));
+  HIPCHK(c, tmp.alloc(&D.order, N));
+  HIPCHK(c, tmp.alloc(&D.cell, N));
+  HIPCHK(c, tmp.alloc(&D.first, N));
+  HIPCHK(c, tmp.alloc(&D.end, N));
+  HIPCHK(c, tmp.alloc(&D.nsorted, 1));
+  launch_debug_order<H>(sc->P, bufs<H>(sc), D, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (dst) HIPCHK(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GDEM_OK;
+  };
+  int rc = 0;
+  rc |= cp(vw->key, D.key, N * 4);
+  rc |= cp(vw->order, D.order, N * 4);
+  rc |= cp(vw->cell, D.cell, N * 4);
+  rc |= cp(vw->cell_first, D.first, N * 4);
+  rc |= cp(vw->cell_end, D.end, N * 4);
+  rc |= cp(vw->n_sorted, D.nsorted, 4);
   return rc ? GDEM_E_HIP : GDEM_OK;
 }
 
@@ -394,6 +485,10 @@ int set_history_impl(gdem_scene* sc, const gdem_history_view* vw) {
 template <class H>
 int stats_impl(gdem_scene* sc, gdem_stats* out) {
   gdem_ctx* c = sc->ctx;
+  {
+    const int rc = recover_barrier<H>(sc, true);
+    if (rc) return rc;
+  }
   HIPCHK(c, hipMemsetAsync(sc->d_acc, 0, 4 * sizeof(unsigned long long), c->stream));
   launch_counts<H>(sc->P, bufs<H>(sc), sc->d_acc, c->stream);
   HIPCHK(c, hipGetLastError());
@@ -450,6 +545,15 @@ int gdem_sync(gdem_ctx* c) {
 }
 
 const char* gdem_last_error(gdem_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gdem_device_arch(int dev, char* buf, int len, int* cus) {
+  if (!buf || len <= 0) return GDEM_E_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return GDEM_E_HIP;
+  std::snprintf(buf, (size_t)len, "%s", prop.gcnArchName);
+  if (cus) *cus = prop.multiProcessorCount;
+  return GDEM_OK;
+}
 
 int gdem_material(int n, const double* r, double density, double young, double cor, double* m_out,
                   double* k_out, double* eta_out) {
@@ -614,6 +718,9 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
   }
   HIPCHK(c, hipEventCreate(&sc->t0));
   HIPCHK(c, hipEventCreate(&sc->t1));
+  HIPCHK(c, hipEventCreateWithFlags(&sc->abort_ev, hipEventDisableTiming));
+  HIPCHK(c, hipHostMalloc((void**)&sc->h_abort, sizeof(uint32_t), hipHostMallocDefault));
+  *sc->h_abort = 0;
   int rc = sc->precision == GDEM_FP64 ? create_impl<double>(sc.get(), pd) : create_impl<float>(sc.get(), pd);
   if (rc) return rc;
   *out = sc.release();
@@ -636,7 +743,16 @@ int gdem_scene_status(gdem_scene* sc, gdem_stats* st) {
                    GDEM_ERR_WHIST_OVERFLOW | GDEM_ERR_VE_OVERFLOW))
     return fail(sc->ctx, GDEM_E_SLOT_OVERFLOW, "slot overflow (raise the capacities in gdem_params)");
   if (o->errors & GDEM_ERR_NAN) return fail(sc->ctx, GDEM_E_NAN, "non-finite particle state");
-  if (o->errors & GDEM_ERR_BARRIER) return fail(sc->ctx, GDEM_E_HIP, "list rebuild: grid barrier timed out");
+  if (o->errors & GDEM_ERR_BARRIER) {
+    // reported once: the scene rebuilds through the launch chain from now on
+    gdem_ctx* c = sc->ctx;
+    int err = 0;
+    HIPCHK(c, hipMemcpy(&err, sc->d_ctl + CTL_ERR, sizeof(int), hipMemcpyDeviceToHost));
+    err &= ~(int)GDEM_ERR_BARRIER;
+    HIPCHK(c, hipMemcpy(sc->d_ctl + CTL_ERR, &err, sizeof(int), hipMemcpyHostToDevice));
+    return fail(sc->ctx, GDEM_E_HIP,
+                "list rebuild: a grid barrier timed out; the scene now rebuilds through per-phase kernels");
+  }
   return GDEM_OK;
 }
 
@@ -654,6 +770,11 @@ void gdem_scene_destroy(gdem_scene* sc) {
 int gdem_scene_debug(gdem_scene* sc, gdem_debug_view* vw) {
   if (!sc || !vw) return GDEM_E_ARG;
   return sc->precision == GDEM_FP64 ? debug_impl<double>(sc, vw) : debug_impl<float>(sc, vw);
+}
+
+int gdem_scene_debug_order(gdem_scene* sc, gdem_order_view* vw) {
+  if (!sc || !vw) return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? debug_order_impl<double>(sc, vw) : debug_order_impl<float>(sc, vw);
 }
 
 int gdem_scene_step_record(gdem_scene* sc, gdem_contact_view* vw) {

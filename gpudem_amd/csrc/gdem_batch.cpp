@@ -472,6 +472,10 @@ int gdem_batch_admit(gdem_batch* b, const gdem_particles_desc* pd, const gdem_me
   sl.nseg = (pd->n + kSlotGranule - 1) / kSlotGranule * kSlotGranule;
   sl.ncell1 = (int64_t)ncell_d + 1;
   sl.ntri = ntri;
+  // a scene larger than the batch's whole capacity can never be admitted:
+  // GDEM_E_OOM, not the retry-later GDEM_E_BUSY
+  if (sl.nseg > b->pcap || sl.ntri > b->tcap)
+    return fail(c, GDEM_E_OOM, "batch: scene larger than the batch's total particle or triangle capacity");
   // reserve the ranges (all or nothing)
   bool ok = b->parts.alloc(sl.nseg, &sl.seg0);
   if (ok && !b->cells.alloc(sl.ncell1, &sl.cell0)) {
@@ -496,7 +500,7 @@ int gdem_batch_admit(gdem_batch* b, const gdem_particles_desc* pd, const gdem_me
     WallCells wc;
     if (!build_wall_cells(mesh, S.bmin, pp->cell, S.size, maxr + pp->skin, (size_t)b->wcap, &wc)) {
       undo();
-      return fail(c, GDEM_E_BUSY, "batch: wall table larger than the free wall capacity");
+      return fail(c, GDEM_E_OOM, "batch: wall table larger than the batch's total wall capacity");
     }
     sl.nwt = (int64_t)wc.tri.size();
     if (!b->wtab.alloc(sl.nwt, &sl.wt0)) {

@@ -68,6 +68,16 @@ struct DebugDev {
   int* nwhist;
 };
 
+// per-slot view of the last rebuild (gdem_order_view)
+struct OrderDev {
+  uint32_t* key;
+  int* order;
+  int* cell;
+  int* first;
+  int* end;
+  int* nsorted;
+};
+
 template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st);
 template <class T, class H>
@@ -89,6 +99,8 @@ void launch_download(const SimParams& P, const Bufs<H>& B, double* x, double* v,
                      hipStream_t st);
 template <class H>
 void launch_debug(const SimParams& P, const Bufs<H>& B, const DebugDev& D, hipStream_t st);
+template <class H>
+void launch_debug_order(const SimParams& P, const Bufs<H>& B, const OrderDev& D, hipStream_t st);
 template <class H>
 void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc, hipStream_t st);
 template <class H>

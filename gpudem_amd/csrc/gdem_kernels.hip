@@ -440,6 +440,9 @@ __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
 template <class H>
 __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
+  // a cancelled grid barrier left the previous lists and the refresh flag in
+  // place for the host to rebuild through the launch chain
+  if (__builtin_amdgcn_readfirstlane(B.gbar[GBAR_ABORT]) != 0) return;
   __shared__ PairLds lds;
   const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
   const int nact = B.ctl[CTL_NACT];
@@ -460,13 +463,20 @@ __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
 // arrivals (about 0.1 us each on MI355X).  Each workgroup releases its writes
 // before arriving and acquires after the generation moves; the counters are
 // reset before the generation word is released, so the next barrier finds
-// them at zero.  The wait is bounded (spin polls, about a second by default):
-// a workgroup whose barrier does not complete sets GDEM_ERR_BARRIER and the
-// sticky abort word, and returns false, and so does every workgroup that
-// finds the abort word set after a barrier; the kernel then ends without
-// using the incomplete phase's data (its scatter and permutation also reject
-// out-of-range positions), and every later rebuild of the scene aborts at
-// once.  The error is reported by gdem_scene_status.
+// them at zero.  The wait is bounded (spin polls, about a second by default).
+// A workgroup whose wait runs out tries to cancel the barrier by setting the
+// generation word's top bit with a compare-and-swap from the value it
+// arrived with; the last arrival advances the word with a compare-and-swap
+// from the same value.  Exactly one of the two succeeds, so either every
+// workgroup passes the barrier or none does: a cancelled barrier sets
+// GDEM_ERR_BARRIER and the sticky abort word, every workgroup returns false,
+// and no phase after it runs.  The state, the lists and the cell of every
+// slot then remain those of the last complete rebuild (the phases before the
+// barrier write only sort buffers and clear cell-table entries that the next
+// rebuild clears again), the refresh flag stays set, and the host switches
+// the scene to the launch chain (gdem_scene_step / gdem_scene_status).
+constexpr uint32_t GBAR_CANCEL = 0x80000000u;
+
 __device__ __forceinline__ bool grid_barrier(uint32_t* gbar, int* ctl, int spin) {
   __shared__ int go;
   __syncthreads();
@@ -479,30 +489,43 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* gbar, int* ctl, int spin)
     uint32_t* top = gbar + BAR_SUB * 32;
     uint32_t* gen = gbar + (BAR_SUB + 1) * 32;
     const uint32_t g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == (uint32_t)members - 1) {
+    int ok = (g0 & GBAR_CANCEL) == 0;  // a barrier of this launch was cancelled
+    if (ok) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const uint32_t prev = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == (uint32_t)members - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == (uint32_t)nsub - 1) {
+          __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          uint32_t expect = g0;
+          __hip_atomic_compare_exchange_strong(gen, &expect, (g0 + 1u) & ~GBAR_CANCEL, __ATOMIC_RELEASE,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      for (int it = 0;; ++it) {
+        const uint32_t gv = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gv != g0) {
+          ok = (gv & GBAR_CANCEL) == 0;
+          break;
+        }
+        if (it >= spin) {
+          uint32_t expect = g0;
+          if (__hip_atomic_compare_exchange_strong(gen, &expect, g0 | GBAR_CANCEL, __ATOMIC_RELAXED,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            atomicOr((unsigned int*)&ctl[CTL_ERR], GDEM_ERR_BARRIER_BIT);
+            __hip_atomic_store(gbar + GBAR_ABORT, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+          } else {
+            ok = (expect & GBAR_CANCEL) == 0;  // completed (or cancelled by another) meanwhile
+          }
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t t = __hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      if (t == (uint32_t)nsub - 1) {
-        __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(gen, g0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
     }
-    uint32_t* abort_word = gbar + GBAR_ABORT;
-    int ok = 1;
-    for (int it = 0; __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g0; ++it) {
-      if (it >= spin) {
-        atomicOr((unsigned int*)&ctl[CTL_ERR], GDEM_ERR_BARRIER_BIT);
-        __hip_atomic_store(abort_word, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) ok = 0;
     go = ok;
   }
   __syncthreads();
@@ -627,6 +650,23 @@ __global__ __launch_bounds__(256) void k_debug(SimParams P, Bufs<H> B, DebugDev 
     D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + 1] = (double)d.y;
     D.whist_delta[((size_t)id * P.whist_cap + q) * 3 + 2] = (double)d.z;
   }
+}
+
+// What the last rebuild stored, per slot: its sorted key, the original id it
+// holds, its cell and that cell's table entry.
+template <class H>
+__global__ __launch_bounds__(256) void k_debug_order(SimParams P, Bufs<H> B, OrderDev D) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.N) return;
+  const int hc = B.ctl[CTL_HCUR];
+  D.key[s] = B.key[RS_FINAL][s];
+  D.order[s] = B.pid[hc][s];
+  const int c = B.cell_of[s];
+  D.cell[s] = c;
+  const int2 e = c >= 0 ? B.cell_tab[c] : make_int2(-1, -1);
+  D.first[s] = e.x;
+  D.end[s] = e.y;
+  if (s == 0) D.nsorted[0] = B.ctl[CTL_NACT];
 }
 
 // inv[original id] = current slot, for k_set_history (inv is free between
@@ -835,6 +875,11 @@ void launch_debug(const SimParams& P, const Bufs<H>& B, const DebugDev& D, hipSt
 }
 
 template <class H>
+void launch_debug_order(const SimParams& P, const Bufs<H>& B, const OrderDev& D, hipStream_t st) {
+  k_debug_order<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, D);
+}
+
+template <class H>
 void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc, hipStream_t st) {
   k_counts<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, acc);
 }
@@ -869,6 +914,8 @@ template void launch_download<double>(const SimParams&, const Bufs<double>&, dou
 template void launch_download<float>(const SimParams&, const Bufs<float>&, double*, double*, double*, int*, hipStream_t);
 template void launch_debug<double>(const SimParams&, const Bufs<double>&, const DebugDev&, hipStream_t);
 template void launch_debug<float>(const SimParams&, const Bufs<float>&, const DebugDev&, hipStream_t);
+template void launch_debug_order<double>(const SimParams&, const Bufs<double>&, const OrderDev&, hipStream_t);
+template void launch_debug_order<float>(const SimParams&, const Bufs<float>&, const OrderDev&, hipStream_t);
 template void launch_counts<double>(const SimParams&, const Bufs<double>&, unsigned long long*, hipStream_t);
 template void launch_counts<float>(const SimParams&, const Bufs<float>&, unsigned long long*, hipStream_t);
 template void launch_set_history<double>(const SimParams&, const Bufs<double>&, const HistIn&, hipStream_t);

@@ -22,10 +22,10 @@ ERRORS = {-1: "GDEM_E_ARG", -2: "GDEM_E_HIP", -3: "GDEM_E_OOM", -4: "GDEM_E_SLOT
           -5: "GDEM_E_NAN", -6: "GDEM_E_STATE", -7: "GDEM_E_BUSY"}
 
 # exported symbols (checked by the CPU test suite against include/gdem.h)
-SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_material",
+SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_device_arch", "gdem_material",
            "gdem_bbox_rule", "gdem_params_default", "gdem_scene_create", "gdem_scene_step",
            "gdem_scene_status", "gdem_scene_download", "gdem_scene_destroy", "gdem_scene_debug",
-           "gdem_scene_step_record", "gdem_scene_timing", "gdem_scene_set_timing",
+           "gdem_scene_step_record", "gdem_scene_debug_order", "gdem_scene_timing", "gdem_scene_set_timing",
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
            "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
            "gdem_batch_step", "gdem_batch_poll", "gdem_batch_download", "gdem_batch_scene_status",
@@ -93,6 +93,10 @@ class DebugView(ctypes.Structure):
                  "hist_delta", "nhist", "whist_key", "whist_delta", "nwhist")]
 
 
+class OrderView(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("key", "order", "cell", "cell_first", "cell_end", "n_sorted")]
+
+
 class ContactView(ctypes.Structure):
     _fields_ = [("rec_cap", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in
                                               ("count", "key", "kind", "fn", "ft", "delta", "accel",
@@ -132,6 +136,7 @@ def lib():
         L.gdem_sync.argtypes = [vp]
         L.gdem_last_error.argtypes = [vp]
         L.gdem_last_error.restype = ctypes.c_char_p
+        L.gdem_device_arch.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, P(ctypes.c_int)]
         L.gdem_material.argtypes = [ctypes.c_int, vp, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double, vp, vp, vp]
         L.gdem_bbox_rule.argtypes = [P(MeshDesc), vp, vp]
@@ -144,6 +149,7 @@ def lib():
         L.gdem_scene_destroy.restype = None
         L.gdem_scene_debug.argtypes = [vp, P(DebugView)]
         L.gdem_scene_step_record.argtypes = [vp, P(ContactView)]
+        L.gdem_scene_debug_order.argtypes = [vp, P(OrderView)]
         L.gdem_scene_timing.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_int)]
         L.gdem_scene_set_timing.argtypes = [vp, ctypes.c_int]
         L.gdem_radix_sort.argtypes = [vp, ctypes.c_int, vp, vp, vp]
@@ -186,6 +192,16 @@ def _ptr(a):
 def _f64(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a.reshape(shape) if shape is not None else a
+
+
+def device_arch(device=0):
+    """Architecture name of a HIP device without its feature suffixes (gfx950)."""
+    buf = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int()
+    rc = lib().gdem_device_arch(int(device), buf, len(buf), ctypes.byref(cus))
+    if rc:
+        raise GdemError(rc, "gdem_device_arch")
+    return buf.value.decode().split(":")[0]
 
 
 def material(r, density, young, cor):
@@ -385,6 +401,19 @@ class Scene:
                  whist_delta=np.empty((n, c["whist"], 3)), nwhist=np.empty(n, np.int32))
         view = DebugView(*[_ptr(d[f]) for f, _ in DebugView._fields_])
         self.ctx.check(lib().gdem_scene_debug(self.h, ctypes.byref(view)), "gdem_scene_debug")
+        return d
+
+    def debug_order(self):
+        """What the last list rebuild stored per slot: sort key, original id,
+        cell id and the cell's (first, end) slot range; plus the number of
+        active (sorted-first) slots."""
+        n = self.n
+        d = dict(key=np.empty(n, np.uint32), order=np.empty(n, np.int32), cell=np.empty(n, np.int32),
+                 cell_first=np.empty(n, np.int32), cell_end=np.empty(n, np.int32),
+                 n_sorted=np.empty(1, np.int32))
+        view = OrderView(*[_ptr(d[f]) for f, _ in OrderView._fields_])
+        self.ctx.check(lib().gdem_scene_debug_order(self.h, ctypes.byref(view)), "gdem_scene_debug_order")
+        d["n_sorted"] = int(d["n_sorted"][0])
         return d
 
     def step_record(self, cap=64):

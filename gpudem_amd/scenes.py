@@ -202,6 +202,65 @@ def helical_chute(seed=3, n=1000000, r0=0.0005, spread=0.1, density=2500., young
     return spec
 
 
+def feature_impact(kind, speed=0.5, r=0.005, side=0.05, dt=1e-6, density=2500., young=1e7, cor=0.5,
+                   mu=0.5, direction=None, gap=1e-5):
+    """One sphere driven head-on, from outside, into a convex mesh feature:
+    kind 'vertex' -- the corner O of three right triangles that meet there at
+    their right angles (faces z=0, x=0 and y=0 of an open corner of side
+    `side`); kind 'edge' -- the edge from O to (0, side, 0) shared by two right
+    triangles (faces z=0 and x=0).  Every triangle at the feature reports it
+    as its closest feature (two barycentric weights negative for the vertex,
+    one for the edge), so the wall pass sees the same vertex or edge id once
+    per triangle and must keep one contact.  A far triangle comes first in the
+    mesh so the feature's triangles have ids 1, 2 (, 3).  The sphere moves
+    along `direction` (a unit vector into the feature), its centre on the
+    line through the contact point, so the contact normal is -direction and
+    no tangential motion arises.  No gravity."""
+    L = side
+    O = np.zeros(3)
+    X, Y, Z = np.array([L, 0., 0.]), np.array([0., L, 0.]), np.array([0., 0., L])
+    far = np.concatenate([[3 * L, 3 * L, 3 * L], [3.5 * L, 3 * L, 3 * L], [3 * L, 3.5 * L, 3 * L]])
+    if kind == "vertex":
+        tris = [far, np.concatenate([O, X, Y]), np.concatenate([O, Y, Z]), np.concatenate([O, Z, X])]
+        d = np.array([1.0, 1.1, 1.25]) if direction is None else np.asarray(direction, np.float64)
+        hit = O
+    elif kind == "edge":
+        tris = [far, np.concatenate([O, Y, X]), np.concatenate([O, Y, Z])]
+        d = np.array([math.cos(0.6), 0.0, math.sin(0.6)]) if direction is None else \
+            np.asarray(direction, np.float64)
+        hit = np.array([0.0, 0.4 * L, 0.0])
+    else:
+        raise ValueError(kind)
+    d = d / np.linalg.norm(d)
+    x = (hit - (r + gap) * d)[None, :]
+    v = (speed * d)[None, :]
+    verts = np.array(tris)
+    lo = np.minimum(x[0] - 2 * r, -2 * r)
+    hi = np.full(3, L)
+    bbox = bbox_rule(verts, [lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]])
+    params = dict(dt=dt, mu=mu, g=(0., 0., 0.), bbox=bbox, cell=4 * r, skin=r, refresh=0.5 * r)
+    spec = SceneSpec(f"{kind}_impact", x, np.array([r]), verts, params, density, young, cor, v=v)
+    spec.direction = d
+    spec.feature_tris = list(range(1, len(tris)))
+    return spec
+
+
+def out_of_bounds_flight(dt=1e-5, g=-9.81):
+    """Six spheres far from one another, no mesh, each flying out of the
+    bounding box [-0.1, 0.1]^3 through a different face (+x, -x, +y, -y under
+    no acceleration along their path, +z against gravity, -z with it).
+    Returns the spec; the exact step at which each one crosses follows from
+    the discrete ballistic sequence x_n = x0 + n v0 dt + g dt^2 n (n+1) / 2."""
+    x = np.array([[0.05, 0.0, 0.0], [-0.05, 0.05, 0.05], [0.0, 0.07, -0.05],
+                  [0.05, -0.06, 0.05], [-0.05, -0.05, 0.06], [0.05, 0.05, -0.04]])
+    v = np.array([[0.71, 0.0, 0.0], [-0.53, 0.0, 0.0], [0.0, 0.37, 0.0],
+                  [0.0, -0.61, 0.0], [0.0, 0.0, 0.9], [0.0, 0.0, -0.2]])
+    r = np.full(len(x), 0.004)
+    bbox = [-0.1, 0.1, -0.1, 0.1, -0.1, 0.1]
+    params = dict(dt=dt, mu=0.5, g=(0., 0., g), bbox=bbox, cell=0.02, skin=0.004, refresh=0.002)
+    return SceneSpec("out_of_bounds", x, r, np.zeros((0, 9)), params, 2500., 1e7, 0.5, v=v)
+
+
 def read_ascii_stl(paths):
     """ASCII STL files -> (verts [ntri,9], tris_per_file), reading facets the
     way load_ascii_stl_double does (TriangleMesh.cu:363-461)."""

@@ -19,7 +19,7 @@ from .engine import Batch
 
 
 class Request:
-    __slots__ = ("rid", "arrival", "spec", "nsteps", "sid", "admitted", "done", "result", "stats")
+    __slots__ = ("rid", "arrival", "spec", "nsteps", "sid", "admitted", "done", "result", "stats", "error")
 
     def __init__(self, rid, arrival, spec, nsteps):
         self.rid = rid
@@ -31,6 +31,7 @@ class Request:
         self.done = None
         self.result = None
         self.stats = None
+        self.error = None
 
 
 class Server:
@@ -50,6 +51,9 @@ class Server:
                            graph_steps=graph_steps)
 
     def _admit(self, req, now):
+        """True if admitted, False if it must wait for capacity; raises
+        GdemError when the request can never be admitted (larger than the
+        batch, or invalid)."""
         s = req.spec
         m, k, e = engine.material(s.r, s.density, s.young, s.cor)
         sid = self.batch.admit(s.x, s.r, m, k, e, s.verts, s.params, req.nsteps, v=s.v)
@@ -83,7 +87,23 @@ class Server:
             # admissions overlap the chunk on the device; they start after it
             while queue:
                 req = queue[0]
-                if not self._admit(req, time.perf_counter() - t0):
+                try:
+                    admitted = self._admit(req, time.perf_counter() - t0)
+                except engine.GdemError as err:
+                    # can never run here (larger than the whole batch, or invalid): fail it
+                    queue.popleft()
+                    req.error = str(err)
+                    req.done = time.perf_counter() - t0
+                    finished.append(req)
+                    continue
+                if not admitted:
+                    if not running:
+                        # refused with nothing running: no eviction will make room
+                        queue.popleft()
+                        req.error = "refused by an empty batch"
+                        req.done = time.perf_counter() - t0
+                        finished.append(req)
+                        continue
                     break
                 queue.popleft()
                 running[req.sid] = req
@@ -111,11 +131,14 @@ class Server:
 
 
 def summarize(finished, makespan):
-    """Throughput and latency percentiles of a finished run."""
+    """Throughput and latency percentiles of a finished run (failed requests
+    are counted apart)."""
+    failed = [r for r in finished if r.error is not None]
+    finished = [r for r in finished if r.error is None]
     lat = np.array([r.done - r.arrival for r in finished])
     wait = np.array([r.admitted - r.arrival for r in finished])
     work = float(sum(r.spec.n * r.nsteps for r in finished))
-    return dict(requests=len(finished), particle_steps=work, makespan_s=makespan,
+    return dict(requests=len(finished), failed=len(failed), particle_steps=work, makespan_s=makespan,
                 value=work / makespan if makespan > 0 else 0.0,
                 p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
                 mean_s=float(lat.mean()), queue_wait_p50_s=float(np.percentile(wait, 50)),

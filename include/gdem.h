@@ -130,6 +130,25 @@ typedef struct {
   int* nwhist;          /* [n] */
 } gdem_debug_view;
 
+/* What the last list rebuild stored, slot by slot (slots are the engine's
+ * internal order after the stable radix sort): the sort key (morton3 of the
+ * cell coordinates d_calcCellId gives, BoundingBox.cu:1007-1023, 1089-1104, or
+ * the cell id when an axis has more than 1024 cells; 0xffffffff for an
+ * inactive particle), the original id the slot holds (the order
+ * cub::DeviceRadixSort::SortPairs leaves, BoundingBox.cu:1235-1242, applied to
+ * the slot order of the previous rebuild), the slot's cell id and the (first,
+ * end) slot range the cell table holds for that cell (the CSR of
+ * dk_build_cellCount / ExclusiveSum / dk_build_pList, BoundingBox.cu:1062-1087,
+ * 1257-1280).  Any pointer may be NULL. */
+typedef struct {
+  uint32_t* key;    /* [n] */
+  int* order;       /* [n] original particle id of each slot */
+  int* cell;        /* [n] cell id of each slot, -1 for an inactive particle */
+  int* cell_first;  /* [n] first slot of the slot's cell, -1 for an inactive particle */
+  int* cell_end;    /* [n] one past the last slot of the slot's cell */
+  int* n_sorted;    /* [1] active slots (they come first) */
+} gdem_order_view;
+
 /* Per-contact record of one step (pairs ascending by partner id, then walls
  * ascending by triangle id): [n*rec_cap] rows. */
 typedef struct {
@@ -182,6 +201,10 @@ int gdem_ctx_create(int hip_device, gdem_ctx** out);
 void gdem_ctx_destroy(gdem_ctx* ctx);
 int gdem_sync(gdem_ctx* ctx);
 const char* gdem_last_error(gdem_ctx* ctx);
+/* Architecture name of a HIP device (gcnArchName, e.g. "gfx950:sramecc+:xnack-")
+ * and its compute-unit count; no counterpart in the reference (the autotuner
+ * keys its cached choices by it). */
+int gdem_device_arch(int hip_device, char* buf, int len, int* cus);
 
 /* ---- set-up helpers ----------------------------------------------------- */
 /* Material of one particle: m = density*3.14*r^3*4/3 (main.cu:67),
@@ -224,6 +247,9 @@ void gdem_scene_destroy(gdem_scene* scene);
 /* Parity hooks: cell coordinates/keys of the current positions, candidate
  * lists and contact histories, all in original particle ids. */
 int gdem_scene_debug(gdem_scene* scene, gdem_debug_view* view);
+/* Parity hook: the keys, slot order and cell table the last list rebuild
+ * stored (see gdem_order_view). */
+int gdem_scene_debug_order(gdem_scene* scene, gdem_order_view* view);
 /* Runs one step that also records every contact's forces. */
 int gdem_scene_step_record(gdem_scene* scene, gdem_contact_view* view);
 /* Replaces the scene's contact histories (snapshot restore: the SAVE_ON
@@ -276,7 +302,9 @@ int gdem_batch_create(gdem_ctx* ctx, const gdem_batch_params* params, gdem_batch
 void gdem_batch_destroy(gdem_batch* batch);
 /* Admits a scene that will run nsteps batch steps from the next one.  Inputs
  * as for gdem_scene_create (params.precision, capacities and graph_steps are
- * the batch's).  GDEM_E_BUSY when no slot or capacity is free now. */
+ * the batch's).  GDEM_E_BUSY when no slot or capacity is free now;
+ * GDEM_E_OOM when the scene exceeds the batch's total capacity (it can never
+ * be admitted); GDEM_E_ARG when its cell grid exceeds the cell capacity. */
 int gdem_batch_admit(gdem_batch* batch, const gdem_particles_desc* particles, const gdem_mesh_desc* mesh,
                      const gdem_params* params, int nsteps, int* scene_out);
 /* Enqueues nsteps batch steps (asynchronous on the context stream). */

@@ -48,6 +48,7 @@ def lib():
         L.orc_sim_set.argtypes = [vp, ctypes.c_char_p, vp]
         L.orc_sim_brute_lists.argtypes = [vp, ip, ip, ip, ip]
         L.orc_sim_morton.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_sim_morton_built.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
         L.orc_morton3.restype = ctypes.c_uint32
         L.orc_morton3.argtypes = [ctypes.c_uint32] * 3
         L.orc_radix_sort.argtypes = [ctypes.POINTER(ctypes.c_uint32), ip, ctypes.c_int]
@@ -186,6 +187,12 @@ class Sim:
     def morton(self):
         out = np.empty(self.N, np.uint32)
         lib().orc_sim_morton(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        return out
+
+    def morton_built(self):
+        """Morton keys of the cells stored at the last rebuild (0xffffffff inactive)."""
+        out = np.empty(self.N, np.uint32)
+        lib().orc_sim_morton_built(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         return out
 
     def history(self, wall=False):

@@ -980,6 +980,17 @@ void orc_sim_morton(void* sp, uint32_t* key) {
   }
 }
 
+// Morton keys of the cell coordinates stored by the last binning (the
+// positions of the last list rebuild); inactive entries get 0xffffffff.
+void orc_sim_morton_built(void* sp, uint32_t* key) {
+  Sim* s = static_cast<Sim*>(sp);
+  for (int i = 0; i < s->N; ++i) {
+    if (s->isActive[i] != 1) { key[i] = 0xffffffffu; continue; }
+    const int* c = &s->cellx[i * 3];
+    key[i] = morton3((uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2]);
+  }
+}
+
 uint32_t orc_morton3(uint32_t ix, uint32_t iy, uint32_t iz) { return morton3(ix, iy, iz); }
 
 void orc_radix_sort(uint32_t* key, int* idx, int n) {

@@ -177,3 +177,28 @@ def test_autotuner_returns_a_timed_candidate(ctx):
                                       candidates=[(0.1, 1.0, 20), (0.3, 1.25, 40)], log=say)
     assert len(table) == 2 and all(t[1] > 0 for t in table)
     assert best in [t[0] for t in table]
+
+
+def test_oversized_request_is_rejected(ctx):
+    # a request larger than the whole batch can never be admitted: the batch
+    # answers GDEM_E_OOM (not the retry-later GDEM_E_BUSY) and the server fails
+    # that request and serves the rest
+    from gpudem_amd import GdemError, serve
+    small = mk(61, 600)
+    big = mk(62, 9000)
+    b = Batch(ctx, precision=GDEM_MIXED, max_scenes=4, particle_capacity=4096, **CAPS)
+    with pytest.raises(GdemError) as err:
+        admit(b, big, 100)
+    assert err.value.code == -3
+    b.close()
+    srv = serve.Server(ctx, GDEM_MIXED, hbm_budget=24 << 20, max_scenes=4, chunk=64, graph_steps=32, **CAPS)
+    cap = srv.batch.info()["free_particles"]
+    big = mk(63, cap + 500)
+    say(f"server of {cap} particle slots with a {big.n}-sphere request")
+    done = srv.run([(0.0, big, 100), (0.01, small, 200)])
+    srv.close()
+    by = {r.rid: r for r in done}
+    assert len(done) == 2
+    assert by[0].error is not None and by[0].result is None
+    assert by[1].error is None and by[1].stats["steps_done"] == 200
+    assert serve.summarize(done, 1.0)["failed"] == 1

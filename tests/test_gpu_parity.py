@@ -44,27 +44,40 @@ def assert_state_equal(sc, orc, what=""):
     assert np.array_equal(d["active"], a), what
 
 
+def _rows_equal(n_eng, rows_eng, n_orc, rows_orc, live, what):
+    """Row counts equal wherever `live`, and the first count entries of every
+    such row equal (vectorised over all particles)."""
+    bad = np.flatnonzero(live & (n_eng != n_orc))
+    assert len(bad) == 0, f"{what}: row lengths differ for {len(bad)} particles, first {bad[:5]}"
+    cap = rows_eng.shape[1]
+    mask = (np.arange(cap)[None, :] < n_eng[:, None]) & live[:, None]
+    diff = (rows_eng != rows_orc[:, :cap])
+    if diff.ndim == 3:
+        diff = np.any(diff, axis=2)
+    bad = np.flatnonzero(np.any(diff & mask, axis=1))
+    assert len(bad) == 0, f"{what}: rows differ for {len(bad)} particles, first {bad[:5]}"
+
+
 def assert_lists_equal(sc, orc):
     dbg = sc.debug()
-    pairs, walls = orc.lists()
-    act = orc.get("isActive")
-    for i in range(orc.N):
-        if act[i] != 1:
-            continue
-        assert np.array_equal(dbg["nb"][i, :dbg["nnb"][i]], pairs[i]), i
-        assert np.array_equal(dbg["wall"][i, :dbg["nwall"][i]], walls[i]), i
+    live = orc.get("isActive") == 1
+    N = orc.N
+    _rows_equal(dbg["nnb"], dbg["nb"], orc.get("numNei"), orc.get("neiList").reshape(N, -1), live, "pair list")
+    _rows_equal(dbg["nwall"], dbg["wall"], orc.get("numNeiWall"), orc.get("neiListWall").reshape(N, -1), live,
+                "wall list")
     return dbg
 
 
 def assert_history_equal(sc, orc):
     dbg = sc.debug()
-    for wall, kk, dd, nn in ((False, "hist_key", "hist_delta", "nhist"),
-                             (True, "whist_key", "whist_delta", "nwhist")):
-        h = orc.history(wall=wall)
-        for i in range(orc.N):
-            n = dbg[nn][i]
-            assert np.array_equal(dbg[kk][i, :n], h[i][0]), (wall, i)
-            assert np.array_equal(dbg[dd][i, :n], h[i][1]), (wall, i)
+    N = orc.N
+    live = np.ones(N, bool)
+    for sfx, kk, dd, nn, names in (("", "hist_key", "hist_delta", "nhist", ("dhx", "dhy", "dhz")),
+                                   ("Wall", "whist_key", "whist_delta", "nwhist", ("dwx", "dwy", "dwz"))):
+        num = orc.get("numCont" + sfx)
+        _rows_equal(dbg[nn], dbg[kk], num, orc.get("indHis" + sfx).reshape(N, -1), live, "history keys" + sfx)
+        d = np.stack([orc.get(n).reshape(N, -1) for n in names], axis=-1)
+        _rows_equal(dbg[nn], dbg[dd], num, d, live, "history deltas" + sfx)
 
 
 # ---------------------------------------------------------------------------
@@ -120,6 +133,33 @@ def test_tri_distance_matches_oracle(ctx):
 # ---------------------------------------------------------------------------
 
 
+def assert_rebuild_store(sc, orc, prev_order):
+    """The keys, slot order and cell table the engine's last rebuild stored
+    equal the oracle's Morton keys (dk_morton_key, BoundingBox.cu:1007-1023)
+    and cells (d_calcCellId, :1089-1104) at the rebuild positions, the stable
+    sort of the previous slot order by those keys (SortPairs, :1235-1242), and
+    the oracle's per-cell counts (CSR, :1062-1087)."""
+    o = sc.debug_order()
+    key_of = orc.morton_built()
+    assert np.array_equal(o["key"], key_of[o["order"]])
+    expect = prev_order[np.argsort(key_of[prev_order], kind="stable")]
+    assert np.array_equal(o["order"], expect)
+    nact = int(np.sum(orc.get("isActive") == 1))
+    assert o["n_sorted"] == nact
+    held = o["order"][:nact]
+    assert np.array_equal(o["cell"][:nact], orc.get("cellId")[held])
+    assert np.all(o["cell"][nact:] == -1) and np.all(o["key"][nact:] == 0xFFFFFFFF)
+    k = o["key"][:nact]
+    assert np.all(np.diff(k.astype(np.int64)) >= 0)
+    starts = np.r_[0, np.flatnonzero(np.diff(k) != 0) + 1]
+    ends = np.r_[starts[1:], nact]
+    run = np.repeat(np.arange(len(starts)), ends - starts)
+    assert np.array_equal(o["cell_first"][:nact], starts[run])
+    assert np.array_equal(o["cell_end"][:nact], ends[run])
+    assert np.array_equal(ends - starts, orc.get("pNum")[o["cell"][starts]])
+    return o["order"].copy()
+
+
 def test_cells_keys_and_lists_at_setup(ctx):
     spec = scenes.polydisperse_bed(n=5000)
     say("scene polydisperse_bed n=5000")
@@ -128,10 +168,10 @@ def test_cells_keys_and_lists_at_setup(ctx):
     assert np.array_equal(dbg["cellx"], orc.get("cellx").reshape(-1, 3))
     assert np.array_equal(dbg["cell_id"], orc.get("cellId"))
     assert np.array_equal(dbg["morton"], orc.morton())
-    # the engine's internal order is the stable sort of those keys
-    k, order = oracle.radix_sort(orc.morton())
-    kk, oo = ctx.radix_sort(orc.morton())
-    assert np.array_equal(order, oo)
+    # what the production rebuild stored: the stable sort of the ids by key
+    order = assert_rebuild_store(sc, orc, np.arange(spec.n, dtype=np.int32))
+    k, ref = oracle.radix_sort(orc.morton_built())
+    assert np.array_equal(order, ref)
 
 
 def test_per_contact_forces_bitwise(ctx):
@@ -177,17 +217,32 @@ def test_lattice_drop_trajectory_bitwise(ctx, verlet):
 
 
 def test_polydisperse_bed_trajectory_bitwise(ctx):
+    # C2: 30,000 polydisperse spheres settling in a box, 5,000 steps bit for
+    # bit; after every rebuild the stored keys, order and cells are checked
     spec = scenes.polydisperse_bed(n=30000)
-    say("scene polydisperse_bed n=30000 600 steps")
+    steps, chunk = 5000, 10
+    say(f"scene polydisperse_bed n=30000 {steps} steps")
     sc, orc = make_pair(ctx, spec)
-    for chunk in range(3):
-        sc.step(200)
-        orc.step(200)
-        assert_state_equal(sc, orc, f"step {(chunk + 1) * 200}")
+    order = assert_rebuild_store(sc, orc, np.arange(spec.n, dtype=np.int32))
+    checked = 0
+    for c in range(steps // chunk):
+        rb = orc.get("rebuilds")
+        sc.step(chunk)
+        orc.step(chunk)
+        if orc.get("rebuilds") == rb + 1:
+            order = assert_rebuild_store(sc, orc, order)
+            checked += 1
+        elif orc.get("rebuilds") > rb + 1:
+            order = sc.debug_order()["order"].copy()
+        if (c + 1) * chunk % 500 == 0:
+            assert_state_equal(sc, orc, f"step {(c + 1) * chunk}")
+    say(f"  rebuilds {orc.get('rebuilds')}, {checked} checked")
+    assert checked >= 3
     assert_lists_equal(sc, orc)
     assert_history_equal(sc, orc)
     st = sc.status()
     assert st["errors"] == 0 and st["rebuilds"] == orc.get("rebuilds")
+    assert st["mean_contacts"] > 0.1  # the lower layers have landed and touch
 
 
 @pytest.mark.parametrize("broadphase", [1, 2])
@@ -230,18 +285,98 @@ def test_spiral_mesh_walls_bitwise(ctx, broadphase):
     assert sc.status()["mean_contacts"] > 0.2
 
 
-def test_inactive_particles_and_out_of_bounds(ctx):
+def test_inactive_particles_stay_put(ctx):
+    # particles flagged inactive at creation take no part in contacts and are
+    # never integrated (cpu_dem.cu:1084-1089, device_dem.cu:1065-1068)
     spec = scenes.lattice_drop(verlet=True)
     act = np.ones(spec.n, np.int32)
     act[::7] = 0
-    v = np.zeros((spec.n, 3))
-    v[3] = [40.0, 0, 0]  # leaves through the wall gap? no: hits the wall; give it out-of-box speed
     m, k, e = engine.material(spec.r, spec.density, spec.young, spec.cor)
-    sc = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, spec.params, v=v, active=act)
-    orc = oracle.Sim(spec.x, spec.r, oracle.Mesh(spec.verts), spec.oracle_params(), v=v, active=act)
+    sc = Scene(ctx, spec.x, spec.r, m, k, e, spec.verts, spec.params, active=act)
+    orc = oracle.Sim(spec.x, spec.r, oracle.Mesh(spec.verts), spec.oracle_params(), active=act)
     sc.step(3000)
     orc.step(3000)
     assert_state_equal(sc, orc, "inactive")
+    d = sc.download()
+    assert np.array_equal(d["active"], act)
+    assert np.array_equal(d["x"][act == 0], spec.x[act == 0])
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_out_of_bounds_on_closed_form_step(ctx, precision):
+    # six spheres leave the box through its six faces; each must turn inactive
+    # on the step the discrete ballistic sequence crosses the face
+    # (dk_checkOoB, device_dem.cu:1116-1125; checkOoB, cpu_dem.cu:1575-1589)
+    from test_oracle_features import crossing_steps
+    spec = scenes.out_of_bounds_flight()
+    steps, margin = crossing_steps(spec)
+    assert np.all(margin > 1e-9)
+    say(f"scene out_of_bounds precision {precision}, crossings at {sorted(steps.tolist())}")
+    sc, orc = make_pair(ctx, spec, precision)
+    done = 0
+    for k in np.sort(steps):
+        for n in (int(k) - 1 - done, 1):
+            sc.step(n)
+            orc.step(n)
+            done += n
+            d = sc.download()
+            assert np.array_equal(d["active"], (steps > done).astype(np.int32)), (done, d["active"])
+            assert np.array_equal(d["active"], orc.get("isActive"))
+            if precision == GDEM_FP64:
+                assert_state_equal(sc, orc, f"step {done}")
+    assert sc.status()["n_active"] == 0
+    x = sc.download()["x"]
+    sc.step(50)
+    assert np.array_equal(sc.download()["x"], x)
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+@pytest.mark.parametrize("kind", ["vertex", "edge"])
+def test_feature_contact_counted_once(ctx, kind, precision):
+    # A sphere driven into a convex vertex (three triangles) or edge (two):
+    # every triangle reports the same feature id, the wall pass keeps the first
+    # (d_wall_collision_verlet, device_dem.cu:288-344), so one contact acts,
+    # its history key is the lowest triangle id, and the rebound follows the
+    # single-contact closed form e^(pi/3.1415).
+    spec = scenes.feature_impact(kind)
+    say(f"scene {kind}_impact precision {precision}")
+    sc, orc = make_pair(ctx, spec, precision)
+    first = min(spec.feature_tris)
+    seen = 0
+    for _ in range(40):
+        sc.step(100)
+        orc.step(100)
+        if precision == GDEM_FP64:
+            assert_state_equal(sc, orc, kind)
+            assert_history_equal(sc, orc)
+        dbg = sc.debug()
+        if dbg["nwhist"][0]:
+            assert dbg["nwhist"][0] == 1 and dbg["whist_key"][0, 0] == first
+            assert sorted(dbg["wall"][0, :dbg["nwall"][0]].tolist()) == spec.feature_tris
+            seen += 1
+    assert seen >= 10
+    v = sc.download()["v"][0]
+    e = np.linalg.norm(v) / np.linalg.norm(spec.v[0])
+    assert e == pytest.approx(0.5 ** (math.pi / 3.1415), rel=2e-3)
+    assert np.allclose(v / np.linalg.norm(v), -spec.direction, atol=1e-6)
+
+
+def test_out_of_bounds_is_strict(ctx):
+    # on the face is inside; the first position beyond it is outside
+    hi = 0.1
+    x = np.array([[hi, 0.0, 0.0], [0.0, hi - 2e-9, 0.0], [0.0, 0.0, 0.05]])
+    v = np.zeros((3, 3))
+    v[1, 1] = 1e-3  # one step of 1e-5 s moves it 1e-8 m, past the face
+    params = dict(dt=1e-5, mu=0.5, g=(0., 0., 0.), bbox=[-0.1, hi, -0.1, hi, -0.1, hi], cell=0.02,
+                  skin=0.004, refresh=0.002)
+    spec = scenes.SceneSpec("strict_oob", x, np.full(3, 0.004), np.zeros((0, 9)), params, 2500., 1e7, 0.5, v=v)
+    sc, orc = make_pair(ctx, spec)
+    sc.step(1)
+    orc.step(1)
+    assert list(sc.download()["active"]) == [1, 0, 1] == list(orc.get("isActive"))
+    sc.step(10)
+    orc.step(10)
+    assert_state_equal(sc, orc, "strict")
 
 
 @pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
