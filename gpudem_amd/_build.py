@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgdem.so")
-SOURCES = ["gdem_kernels.hip", "gdem_batch.hip", "gdem_api.cpp", "gdem_batch.cpp", "gdem_mesh.cpp",
+SOURCES = ["gdem_kernels.hip", "gdem_batch.hip", "gdem_noalloc.hip", "gdem_api.cpp", "gdem_batch.cpp", "gdem_mesh.cpp",
            "gdem_settings.cpp", "gdem_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

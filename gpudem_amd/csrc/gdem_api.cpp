@@ -528,6 +528,12 @@ int gdem_ctx_create(int hip_device, gdem_ctx** out) {
     delete c;
     return GDEM_E_HIP;
   }
+  // the allocation-free download's staging array, once per device
+  if (noalloc_init(hip_device) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GDEM_E_HIP;
+  }
   *out = c;
   return GDEM_OK;
 }
@@ -753,6 +759,27 @@ int gdem_scene_status(gdem_scene* sc, gdem_stats* st) {
     return fail(sc->ctx, GDEM_E_HIP,
                 "list rebuild: a grid barrier timed out; the scene now rebuilds through per-phase kernels");
   }
+  return GDEM_OK;
+}
+
+int gdem_scene_download_noalloc(gdem_scene* sc, gdem_particles_out* out) {
+  if (!sc || !out) return GDEM_E_ARG;
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  if (N > (size_t)NOALLOC_MAX_N)
+    return fail(c, GDEM_E_ARG, "gdem_scene_download_noalloc: more than 262144 particles (use gdem_scene_download)");
+  const NoallocStage S = noalloc_stage(c->device);
+  if (!S.x) return fail(c, GDEM_E_STATE, "gdem_scene_download_noalloc: staging not set up on this device");
+  if (sc->precision == GDEM_FP64)
+    launch_download<double>(sc->P, sc->b64, S.x, S.v, S.w, S.act, c->stream);
+  else
+    launch_download<float>(sc->P, sc->b32, S.x, S.v, S.w, S.act, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out->x) HIPCHK(c, hipMemcpy(out->x, S.x, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->v) HIPCHK(c, hipMemcpy(out->v, S.v, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->angv) HIPCHK(c, hipMemcpy(out->angv, S.w, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (out->is_active) HIPCHK(c, hipMemcpy(out->is_active, S.act, N * 4, hipMemcpyDeviceToHost));
   return GDEM_OK;
 }
 

@@ -68,6 +68,20 @@ struct DebugDev {
   int* nwhist;
 };
 
+// Allocation-free download: the staging array (gdem_noalloc.hip) holds up to
+// NOALLOC_MAX_N particles.
+constexpr int NOALLOC_MAX_N = 1 << 18;
+struct NoallocStage {
+  double* x;  // [NOALLOC_MAX_N * 3]
+  double* v;
+  double* w;
+  int* act;   // [NOALLOC_MAX_N]
+};
+// Resolves the staging array's address on `device`, which must be current.
+hipError_t noalloc_init(int device);
+// The staging pointers of `device` (null members before noalloc_init).
+NoallocStage noalloc_stage(int device);
+
 // per-slot view of the last rebuild (gdem_order_view)
 struct OrderDev {
   uint32_t* key;

@@ -24,7 +24,7 @@ ERRORS = {-1: "GDEM_E_ARG", -2: "GDEM_E_HIP", -3: "GDEM_E_OOM", -4: "GDEM_E_SLOT
 # exported symbols (checked by the CPU test suite against include/gdem.h)
 SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_device_arch", "gdem_material",
            "gdem_bbox_rule", "gdem_params_default", "gdem_scene_create", "gdem_scene_step",
-           "gdem_scene_status", "gdem_scene_download", "gdem_scene_destroy", "gdem_scene_debug",
+           "gdem_scene_status", "gdem_scene_download", "gdem_scene_download_noalloc", "gdem_scene_destroy", "gdem_scene_debug",
            "gdem_scene_step_record", "gdem_scene_debug_order", "gdem_scene_timing", "gdem_scene_set_timing",
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
            "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
@@ -145,6 +145,7 @@ def lib():
         L.gdem_scene_step.argtypes = [vp, ctypes.c_int]
         L.gdem_scene_status.argtypes = [vp, P(Stats)]
         L.gdem_scene_download.argtypes = [vp, P(ParticlesOut)]
+        L.gdem_scene_download_noalloc.argtypes = [vp, P(ParticlesOut)]
         L.gdem_scene_destroy.argtypes = [vp]
         L.gdem_scene_destroy.restype = None
         L.gdem_scene_debug.argtypes = [vp, P(DebugView)]
@@ -389,6 +390,17 @@ class Scene:
         out = ParticlesOut(_ptr(x), _ptr(v), _ptr(w), _ptr(a))
         self.ctx.check(lib().gdem_scene_download(self.h, ctypes.byref(out)), "gdem_scene_download")
         return dict(x=x, v=v, angv=w, active=a)
+
+    def download_noalloc(self, out=None):
+        """download() through the library's fixed staging array (no device
+        allocation; scenes of at most 262,144 particles).  `out`: a dict of
+        preallocated arrays as download() returns, filled in place."""
+        if out is None:
+            out = dict(x=np.empty((self.n, 3)), v=np.empty((self.n, 3)), angv=np.empty((self.n, 3)),
+                       active=np.empty(self.n, np.int32))
+        po = ParticlesOut(_ptr(out["x"]), _ptr(out["v"]), _ptr(out["angv"]), _ptr(out["active"]))
+        self.ctx.check(lib().gdem_scene_download_noalloc(self.h, ctypes.byref(po)), "gdem_scene_download_noalloc")
+        return out
 
     def debug(self):
         n, c = self.n, self.caps

@@ -241,6 +241,14 @@ int gdem_scene_step(gdem_scene* scene, int nsteps);
 int gdem_scene_status(gdem_scene* scene, gdem_stats* stats);
 /* Replaces copyFromDevice (ParticleSystem.h:99, SAVE_ON members). */
 int gdem_scene_download(gdem_scene* scene, gdem_particles_out* out);
+/* No counterpart in the reference.  gdem_scene_download without any device
+ * allocation, for callers that cannot allocate (code between hipGraph
+ * captures, tight serving loops, harnesses that forbid hipMalloc): the
+ * gather into original order goes through a fixed staging array that each
+ * GPU holds for the life of the process (19 MiB).  Scenes of at most 262,144
+ * particles (GDEM_E_ARG beyond).  Calls that overlap on one GPU share the
+ * array and race. */
+int gdem_scene_download_noalloc(gdem_scene* scene, gdem_particles_out* out);
 /* Replaces deallocate(ParticleSys<DeviceMemory>*) and the free_* calls.
  * Destroy every scene before the context it was created on. */
 void gdem_scene_destroy(gdem_scene* scene);
