@@ -303,6 +303,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (cached or default settings)")
     ap.add_argument("--batch-graph-steps", type=int, default=None,
                     help="override the tuned steps per hipGraph of the batched run (0: direct launches)")
+    ap.add_argument("--save-state", help="write the chute's state and contact histories at the start of "
+                                         "the timed window to this .npz (for profiling runs)")
+    ap.add_argument("--load-state", help="start the chute from this .npz (written by --save-state) instead "
+                                         "of settling it")
     args = ap.parse_args()
     if args.cpu_leg:
         cpu_leg(args.cpu_leg, args.cpu_steps, args.cpu_warm)
@@ -322,18 +326,32 @@ def main():
     copy_ms = ctx.copy_probe(copy_bytes, 20)
     copy_bw = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
 
-    # 1. settle (mixed), keeping state and histories
-    sc = chute_scene(ctx, spec, GDEM_MIXED)
-    t0 = time.perf_counter()
-    sc.step(args.settle)
-    ctx.sync()
-    snap = sc.snapshot()
-    st = sc.status()
-    log(f"# settled {args.settle} steps in {time.perf_counter() - t0:.1f} s: contacts/particle "
-        f"{st['mean_contacts']:.2f}, candidates {st['mean_nb']:.2f} + {st['mean_wall']:.2f}, "
-        f"{st['rebuilds']} rebuilds")
-    # 2-3. warm-up and timed window, mixed (the same scene continues)
+    import numpy as np
     K, W, blk = args.steps, args.warmup, args.block
+    if args.load_state:
+        z = np.load(args.load_state)
+        snap = {k: z[k] for k in z.files}
+        sc = chute_scene(ctx, spec, GDEM_MIXED, snap)
+        log(f"# loaded the settled state from {args.load_state}")
+    else:
+        # 1. settle (mixed), keeping state and histories
+        sc = chute_scene(ctx, spec, GDEM_MIXED)
+        t0 = time.perf_counter()
+        sc.step(args.settle)
+        ctx.sync()
+        snap = sc.snapshot()
+        st = sc.status()
+        log(f"# settled {args.settle} steps in {time.perf_counter() - t0:.1f} s: contacts/particle "
+            f"{st['mean_contacts']:.2f}, candidates {st['mean_nb']:.2f} + {st['mean_wall']:.2f}, "
+            f"{st['rebuilds']} rebuilds")
+    if args.save_state:
+        sc.step(W * blk)
+        np.savez(args.save_state, **sc.snapshot())
+        log(f"# saved the state after {W * blk} warm-up steps to {args.save_state}")
+        sc.close()
+        ctx.close()
+        return
+    # 2-3. warm-up and timed window, mixed (the same scene continues)
     mixed = timed_window(sc, W * blk, K * blk)
     sc.close()
     m = summarize(mixed, K * blk, True)

@@ -30,6 +30,9 @@ FLAGS = [
 ]
 
 
+DEMO = os.path.join(HERE, "gdem_compat_demo")
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -38,6 +41,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    # the C++ caller of include/gdem_compat.h (host compiler only, links libgdem.so)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "compat_demo.cpp"), "-o", DEMO + ".tmp", "-L", HERE, "-lgdem",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(DEMO + ".tmp", DEMO)
     return LIB
 
 

@@ -249,13 +249,14 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   // radix-sort / rebuild grid: one workgroup per chunk of >= 256 keys
   SimParams& P = sc->P;
   const int want = (N + 255) / 256;
-  // The rebuild kernel's grid stays well below the co-residency the
-  // occupancy query reports (half of it, at most 1024 workgroups): every
-  // workgroup must be resident at once for its grid barriers (a normal
-  // launch of a grid that fits is resident once the preceding kernel has
-  // drained; the barrier's bounded wait turns anything else into an error),
-  // and the rebuild phases are memory-bound loops that do not need more.
-  const int limit = sc->chain ? 0 : std::min(1024, coop_grid_limit<H>() / 2);
+  // The rebuild kernel's grid is at most the co-residency the occupancy query
+  // reports (two workgroups per CU: the pair lists' LDS columns), and at most
+  // 1024 workgroups: every workgroup must be resident at once for its grid
+  // barriers.  A normal launch of a grid that fits is resident once the
+  // preceding kernel has drained; if other work holds the device for longer
+  // than the barrier's bounded wait, the barrier is cancelled and the scene
+  // continues on the launch chain.
+  const int limit = sc->chain ? 0 : std::min(1024, coop_grid_limit<H>());
   sc->coop = limit > 0;
   sc->chain_blocks = std::max(1, std::min(1024, (N + 2047) / 2048));
   P.rs_blocks = sc->coop ? std::max(1, std::min(limit, want)) : sc->chain_blocks;

@@ -321,6 +321,100 @@ int batch_download_impl(gdem_batch* b, int sid, gdem_particles_out* out) {
   return GDEM_OK;
 }
 
+// Contact histories of one scene in its particles' own ids: the slot rows of
+// the scene's segment, the ids of its slots and the buffer parity are
+// copied back and translated on the host (pair keys are partner slots on the
+// device, wall keys global triangle ids).
+template <class H>
+int batch_history_impl(gdem_batch* b, int sid, const gdem_history_out* out) {
+  gdem_ctx* c = b->ctx;
+  BBufs<H>& B = bbufs<H>(b);
+  const Slot& sl = b->slots[sid];
+  const size_t ns = (size_t)sl.nseg, N = (size_t)b->pcap, n = (size_t)sl.n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  SlotCtl ctl;
+  HIPCHK(c, hipMemcpy(&ctl, b->d_sctl + sid, sizeof(ctl), hipMemcpyDeviceToHost));
+  const int q = ctl.q;
+  std::vector<int> pid(ns), nh(ns), nw(ns), hk((size_t)b->hist_cap * ns), wk((size_t)b->whist_cap * ns);
+  std::vector<Del3<H>> hd((size_t)b->hist_cap * ns), wd((size_t)b->whist_cap * ns);
+  HIPCHK(c, hipMemcpy(pid.data(), B.pid[q] + sl.seg0, ns * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nh.data(), B.nh[q] + sl.seg0, ns * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(nw.data(), B.nw[q] + sl.seg0, ns * 4, hipMemcpyDeviceToHost));
+  for (int k = 0; k < b->hist_cap; ++k) {
+    HIPCHK(c, hipMemcpy(hk.data() + k * ns, B.hkey[q] + k * N + sl.seg0, ns * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hd.data() + k * ns, B.hd[q] + k * N + sl.seg0, ns * sizeof(Del3<H>), hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < b->whist_cap; ++k) {
+    HIPCHK(c, hipMemcpy(wk.data() + k * ns, B.wkey[q] + k * N + sl.seg0, ns * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(wd.data() + k * ns, B.wd[q] + k * N + sl.seg0, ns * sizeof(Del3<H>), hipMemcpyDeviceToHost));
+  }
+  for (size_t i = 0; i < ns; ++i) {
+    const int id = pid[i];
+    if (id < 0 || (size_t)id >= n) continue;
+    out->nhist[id] = nh[i];
+    for (int k = 0; k < nh[i]; ++k) {
+      const int j = hk[k * ns + i] - (int)sl.seg0;  // partner slot -> its id
+      const size_t o = (size_t)id * b->hist_cap + k;
+      out->hist_key[o] = (j >= 0 && (size_t)j < ns) ? pid[j] : -1;
+      const Del3<H> d = hd[k * ns + i];
+      out->hist_delta[o * 3] = (double)d.x;
+      out->hist_delta[o * 3 + 1] = (double)d.y;
+      out->hist_delta[o * 3 + 2] = (double)d.z;
+    }
+    out->nwhist[id] = nw[i];
+    for (int k = 0; k < nw[i]; ++k) {
+      const size_t o = (size_t)id * b->whist_cap + k;
+      out->whist_key[o] = wk[k * ns + i] - (int)sl.tri0;
+      const Del3<H> d = wd[k * ns + i];
+      out->whist_delta[o * 3] = (double)d.x;
+      out->whist_delta[o * 3 + 1] = (double)d.y;
+      out->whist_delta[o * 3 + 2] = (double)d.z;
+    }
+  }
+  return GDEM_OK;
+}
+
+// Histories into a scene admitted and not stepped yet: its slots still hold
+// the particles in their own order (slot seg0 + id, buffer parity 0), so
+// the rows are written straight in device form.
+template <class H>
+int batch_set_history_impl(gdem_batch* b, int sid, const gdem_history_view* in) {
+  BBufs<H>& B = bbufs<H>(b);
+  const Slot& sl = b->slots[sid];
+  const size_t ns = (size_t)sl.nseg, N = (size_t)b->pcap, n = (size_t)sl.n;
+  std::vector<int> nh(ns, 0), nw(ns, 0), hk((size_t)b->hist_cap * ns, -1), wk((size_t)b->whist_cap * ns, -1);
+  std::vector<Del3<H>> hd((size_t)b->hist_cap * ns, Del3<H>{}), wd((size_t)b->whist_cap * ns, Del3<H>{});
+  for (size_t i = 0; i < n; ++i) {
+    nh[i] = in->nhist[i];
+    for (int k = 0; k < nh[i]; ++k) {
+      const size_t o = i * b->hist_cap + k;
+      const int j = in->hist_key[o];
+      hk[k * ns + i] = (j >= 0 && (size_t)j < n) ? (int)sl.seg0 + j : -1;
+      hd[k * ns + i] = Del3<H>{(H)in->hist_delta[o * 3], (H)in->hist_delta[o * 3 + 1], (H)in->hist_delta[o * 3 + 2]};
+    }
+    nw[i] = in->nwhist[i];
+    for (int k = 0; k < nw[i]; ++k) {
+      const size_t o = i * b->whist_cap + k;
+      wk[k * ns + i] = in->whist_key[o] + (int)sl.tri0;
+      wd[k * ns + i] = Del3<H>{(H)in->whist_delta[o * 3], (H)in->whist_delta[o * 3 + 1], (H)in->whist_delta[o * 3 + 2]};
+    }
+  }
+  int rc = 0;
+  rc |= upload(b, B.nh[0] + sl.seg0, nh.data(), ns * 4);
+  rc |= upload(b, B.nw[0] + sl.seg0, nw.data(), ns * 4);
+  for (int k = 0; k < b->hist_cap; ++k) {
+    rc |= upload(b, B.hkey[0] + k * N + sl.seg0, hk.data() + k * ns, ns * 4);
+    rc |= upload(b, B.hd[0] + k * N + sl.seg0, hd.data() + k * ns, ns * sizeof(Del3<H>));
+  }
+  for (int k = 0; k < b->whist_cap; ++k) {
+    rc |= upload(b, B.wkey[0] + k * N + sl.seg0, wk.data() + k * ns, ns * 4);
+    rc |= upload(b, B.wd[0] + k * N + sl.seg0, wd.data() + k * ns, ns * sizeof(Del3<H>));
+  }
+  if (rc) return fail(b->ctx, GDEM_E_HIP, "batch: history upload failed");
+  HIPCHK(b->ctx, hipEventRecord(b->up, b->ctx->stream));
+  return GDEM_OK;
+}
+
 template <class H>
 int batch_counts_impl(gdem_batch* b, int sid, unsigned long long acc[4]) {
   gdem_ctx* c = b->ctx;
@@ -582,6 +676,32 @@ int gdem_batch_download(gdem_batch* b, int sid, gdem_particles_out* out) {
   if (!valid_slot(b, sid) || !out) return GDEM_E_ARG;
   return b->precision == GDEM_FP64 ? batch_download_impl<double>(b, sid, out)
                                    : batch_download_impl<float>(b, sid, out);
+}
+
+int gdem_batch_history(gdem_batch* b, int sid, const gdem_history_out* out) {
+  if (!valid_slot(b, sid) || !out || !out->nhist || !out->hist_key || !out->hist_delta || !out->nwhist ||
+      !out->whist_key || !out->whist_delta)
+    return GDEM_E_ARG;
+  return b->precision == GDEM_FP64 ? batch_history_impl<double>(b, sid, out) : batch_history_impl<float>(b, sid, out);
+}
+
+int gdem_batch_set_history(gdem_batch* b, int sid, const gdem_history_view* in) {
+  if (!valid_slot(b, sid) || !in || !in->nhist || !in->hist_key || !in->hist_delta || !in->nwhist ||
+      !in->whist_key || !in->whist_delta)
+    return GDEM_E_ARG;
+  const Slot& sl = b->slots[sid];
+  if (b->step != sl.start)
+    return fail(b->ctx, GDEM_E_STATE, "batch: histories are set between admission and the next batch step");
+  for (int i = 0; i < sl.n; ++i) {
+    if (in->nhist[i] < 0 || in->nhist[i] > b->hist_cap || in->nwhist[i] < 0 || in->nwhist[i] > b->whist_cap)
+      return fail(b->ctx, GDEM_E_SLOT_OVERFLOW, "history row longer than the batch's capacities");
+    for (int k = 0; k < in->nwhist[i]; ++k) {
+      const int t = in->whist_key[(size_t)i * b->whist_cap + k];
+      if (t < 0 || t >= sl.ntri) return fail(b->ctx, GDEM_E_ARG, "wall history key is not a triangle of the scene");
+    }
+  }
+  return b->precision == GDEM_FP64 ? batch_set_history_impl<double>(b, sid, in)
+                                   : batch_set_history_impl<float>(b, sid, in);
 }
 
 int gdem_batch_scene_status(gdem_batch* b, int sid, gdem_batch_scene_stats* st) {

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
 
 // Pair and wall candidate lists of the permuted scene (the single-scene list
 // builders with this scene's grid), then the scene's buffer parity flips.
-template <class H>
+template <class H, bool LONG>
 __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   __shared__ PairLds lds;
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
@@ -222,8 +222,8 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   L.ntri = S.ntri;
   L.wall_table = 1;
   const bool active = (i - S.seg0) < C.nact;
-  pairlist_one(L, i, active, lds);
-  walllist_one(L, i, active);
+  pairlist_one<PL_BLOCK, LONG>(L, i, active, lds);
+  walllist_one<false>(L, i, active);  // batch scenes always have the per-cell table
   // every later reader of q runs in a later kernel; this kernel reads q2 only
   if (__lane_id() == 0) C.q = qn;
   if (i == S.seg0) C.rebuilds += 1;
@@ -373,7 +373,8 @@ void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st) {
   kb_bin<H><<<grid, 256, 0, st>>>(B);
   kb_scan<H><<<nslots, 256, 0, st>>>(B);
   kb_permute<H><<<grid, 256, 0, st>>>(B);
-  kb_lists<H><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
+  if (B.nb_cap <= PL_LDS_CAP) kb_lists<H, false><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
+  else kb_lists<H, true><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
   if (B.hist_cap <= BPCAP_LDS && B.whist_cap <= BWCAP_LDS)
     kb_step<T, H, HistLds<H, BPCAP_LDS>, HistLds<H, BWCAP_LDS>><<<grid, 256, 0, st>>>(B);
   else

@@ -1437,15 +1437,19 @@ __device__ __forceinline__ void pair_candidates(const ListArgs& L, int s, const 
 }
 
 // Lists up to PL_LDS_CAP long are gathered in the thread's LDS column
-// (col[k * PL_BLOCK]), one store per accepted candidate, and sorted there once
-// at the end; longer capacities use private arrays.
+// (col[k * COLS], COLS the workgroup's thread count), one store per accepted
+// candidate, and sorted there once at the end; longer capacities use private
+// arrays.
 constexpr int PL_BLOCK = 128;   // threads per workgroup of the list kernels
 constexpr int PL_LDS_CAP = 32;
 
-struct PairLds {
-  int2 e[PL_LDS_CAP][PL_BLOCK];
+template <int COLS>
+struct PairLdsT {
+  int2 e[PL_LDS_CAP][COLS];
 };
+using PairLds = PairLdsT<PL_BLOCK>;
 
+template <int COLS>
 __device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col) {
   const int N = L.N;
   const double4 p = L.pos[s];
@@ -1457,21 +1461,21 @@ __device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col
       ovf = true;
       return;
     }
-    col[n * PL_BLOCK] = make_int2(q, id);
+    col[n * COLS] = make_int2(q, id);
     ++n;
   });
   // ids are distinct, so the order is total
   for (int i = 1; i < n; ++i) {
-    const int2 x = col[i * PL_BLOCK];
+    const int2 x = col[i * COLS];
     int j = i - 1;
     for (; j >= 0; --j) {
-      const int2 y = col[j * PL_BLOCK];
+      const int2 y = col[j * COLS];
       if (y.y < x.y) break;
-      col[(j + 1) * PL_BLOCK] = y;
+      col[(j + 1) * COLS] = y;
     }
-    col[(j + 1) * PL_BLOCK] = x;
+    col[(j + 1) * COLS] = x;
   }
-  for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * PL_BLOCK].x;
+  for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * COLS].x;
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
 }
@@ -1504,8 +1508,11 @@ __device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
 }
 
-// lds: the workgroup's PairLds (threads of a PL_BLOCK-thread workgroup)
-__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active, PairLds& lds) {
+// lds: the workgroup's list columns (one per thread of a COLS-thread
+// workgroup).  LONG: capacities above PL_LDS_CAP, gathered in private arrays
+// (a separate kernel instantiation, so the usual one carries no scratch).
+template <int COLS, bool LONG>
+__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active, PairLdsT<COLS>& lds) {
   {
     const double4 p = L.pos[s];
     L.ref[s] = p;
@@ -1515,8 +1522,8 @@ __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool acti
     L.nnb[s] = 0;
     return;
   }
-  if (L.nb_cap <= PL_LDS_CAP) pairlist_lds(L, s, &lds.e[0][threadIdx.x]);
-  else pairlist_arrays(L, s);
+  if constexpr (LONG) pairlist_arrays(L, s);
+  else pairlist_lds<COLS>(L, s, &lds.e[0][threadIdx.x]);
 }
 
 // Wall candidates by BVH traversal (k_update_neighborlist_wall,
@@ -1524,6 +1531,10 @@ __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool acti
 // dist < r + skin, ascending triangle ids.
 constexpr int WL_MAX = 48;
 
+// BVH: traversal (its stack and sort buffer are private arrays, so it is a
+// kernel instantiation of its own); otherwise the per-cell table, which every
+// scene with triangles and no BVH request has.
+template <bool BVH>
 __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool active) {
   const int N = L.N;
   if (!active || L.ntri == 0) {
@@ -1534,9 +1545,6 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
   const double R = p.w + L.skin;
   const double Rsq = R * R;
   const int cap = L.wall_cap < WL_MAX ? L.wall_cap : WL_MAX;
-  int list[WL_MAX];
-  int n = 0;
-  bool ovf = false;
   // sphereAABBOverlapNeighbor (BVH.cu:542-574) then the exact test
   auto consider = [&](int t, const double* lo, const double* hi) {
     double ex = 0.0, ey = 0.0, ez = 0.0;
@@ -1548,10 +1556,12 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
     else if (p.z > hi[2]) ez = p.z - hi[2];
     return (ex * ex + ey * ey + ez * ez) <= Rsq;
   };
-  if (L.wall_table) {
+  if constexpr (!BVH) {
     // static per-cell table: ascending triangle ids already, so candidates
     // go straight to their slots
     // (entries and their boxes are loaded WL_BATCH at a time)
+    int n = 0;
+    bool ovf = false;
     constexpr int WL_BATCH = 4;
     const int c = L.cell_of[s];
     const int b0 = L.wc_off[c], b1 = L.wc_off[c + 1];
@@ -1584,8 +1594,10 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
     }
     L.nwl[s] = n;
     if (ovf) atomicOr(L.err, GDEM_ERR_WALL_OVERFLOW_BIT);
-    return;
   } else {
+    int list[WL_MAX];
+    int n = 0;
+    bool ovf = false;
     int stack[64];
     int sp = 0;
     stack[sp++] = 0;
@@ -1617,10 +1629,10 @@ __device__ __forceinline__ void walllist_one(const ListArgs& L, int s, bool acti
         }
       }
     }
+    for (int q = 0; q < n; ++q) L.wl[(size_t)q * N + s] = list[q];
+    L.nwl[s] = n;
+    if (ovf) atomicOr(L.err, GDEM_ERR_WALL_OVERFLOW_BIT);
   }
-  for (int q = 0; q < n; ++q) L.wl[(size_t)q * N + s] = list[q];
-  L.nwl[s] = n;
-  if (ovf) atomicOr(L.err, GDEM_ERR_WALL_OVERFLOW_BIT);
 }
 
 }  // namespace gdem

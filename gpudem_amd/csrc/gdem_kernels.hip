@@ -423,21 +423,21 @@ __device__ __forceinline__ ListArgs list_args(const SimParams& P, const Bufs<H>&
 
 // Cell table, and the wall list of each slot (it needs only the slot's own
 // permuted record and the static triangle tables).
-template <class H>
+template <class H, bool BVH>
 __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < P.N) {
     cells_one(P, B, s);
     const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
-    walllist_one(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
+    walllist_one<BVH>(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
   }
 }
 
 // Pair lists of the sorted particles, at full occupancy (both the chain and
 // the grid-barrier rebuild end here).  The last workgroup clears the refresh
 // flag, which ends the rebuild.
-template <class H>
+template <class H, bool LONG>
 __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   // a cancelled grid barrier left the previous lists and the refresh flag in
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
   const int nact = B.ctl[CTL_NACT];
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.N; s += gridDim.x * blockDim.x)
-    pairlist_one(L, s, s < nact, lds);
+    pairlist_one<PL_BLOCK, LONG>(L, s, s < nact, lds);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -532,13 +532,16 @@ __device__ __forceinline__ bool grid_barrier(uint32_t* gbar, int* ctl, int spin)
   return go != 0;
 }
 
-// Keys, the three radix passes, the permutation and the cell table need grid
-// barriers; they run in one kernel launched every step with a co-resident
+// The whole rebuild in one kernel launched every step with a co-resident
 // grid, which returns at once (before any barrier) when the refresh flag is
-// clear.  Every workgroup reads the control words before the
-// first barrier; workgroup 0 flips the buffers at the end.  The lists follow
-// in k_lists.
-template <class H>
+// clear: keys (counted for the first radix pass as they are made), the three
+// radix passes, the permutation, the cell table and each slot's wall list,
+// then -- after one more barrier, once every slot's permuted record and cell
+// entry exist -- the pair lists.  One launch per step instead of two halves
+// the cost of the steps that rebuild nothing.  Every workgroup reads the
+// control words before the first barrier; the last workgroup to finish
+// flips the buffers, counts the rebuild and clears the refresh flag.
+template <class H, bool LONG, bool BVH>
 __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   if (__builtin_amdgcn_readfirstlane(B.gbar[GBAR_ABORT]) != 0) return;  // an earlier barrier timed out
@@ -551,8 +554,16 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   const int first = b * blockDim.x + threadIdx.x;
   uint32_t* hist = B.rs_hist;
   uint32_t* dtot = B.rs_hist + (size_t)RS_DIGITS * G;
-  __shared__ uint32_t dig[RS_DIGITS];  // digit counts, then scatter bases
-  __shared__ uint16_t wcnt[RS_WAVES * RS_DIGITS];
+  // the sort's counts and bases, and later the pair lists' columns, share LDS
+  __shared__ union {
+    struct {
+      uint32_t dig[RS_DIGITS];  // digit counts, then scatter bases
+      uint16_t wcnt[RS_WAVES * RS_DIGITS];
+    } rs;
+    PairLdsT<256> pl;
+  } lds;
+  uint32_t* dig = lds.rs.dig;
+  uint16_t* wcnt = lds.rs.wcnt;
   // keys of this workgroup's chunk, counted for the first pass as they are made
   rs_clear(dig);
   {
@@ -579,16 +590,21 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   // the cell of each slot, which this thread's own permute step wrote; so
   // does the wall list, with the slot's permuted position.
   for (int s = first; s < N; s += stride) permute_one(P, B, s, cur, hc);
-  {
-    const ListArgs L = list_args(P, B, cur ^ 1, hc ^ 1);
-    for (int s = first; s < N; s += stride) {
-      cells_one(P, B, s);
-      walllist_one(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
-    }
+  const ListArgs L = list_args(P, B, cur ^ 1, hc ^ 1);
+  for (int s = first; s < N; s += stride) {
+    cells_one(P, B, s);
+    walllist_one<BVH>(L, s, B.key[RS_FINAL][s] != 0xffffffffu);
   }
-  if (b == 0 && threadIdx.x == 0) {
+  // pair lists: every slot's neighbour cells and records are in place
+  if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
+  const int nact = B.ctl[CTL_NACT];
+  for (int s = first; s < N; s += stride) pairlist_one<256, LONG>(L, s, s < nact, lds.pl);
+  if (last_block(&B.ctl[CTL_DONE_WALL])) {
+    B.ctl[CTL_DONE_WALL] = 0;
+    B.ctl[CTL_REBUILDS] += 1;
     B.ctl[CTL_CUR] = cur ^ 1;
     B.ctl[CTL_HCUR] = hc ^ 1;
+    atomicExch(&B.ctl[CTL_REFRESH], 0);
   }
 }
 
@@ -830,7 +846,7 @@ int list_grid(int n) {
   int dev = 0, per_cu = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return want;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lists<H>, PL_BLOCK, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_lists<H, false>, PL_BLOCK, 0) != hipSuccess || per_cu < 1)
     return want;
   return std::max(1, std::min(want, per_cu * prop.multiProcessorCount));
 }
@@ -840,7 +856,7 @@ int coop_grid_limit() {
   int dev = 0, per_cu = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_grid<H>, 256, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rebuild_grid<H, false, false>, 256, 0) != hipSuccess) return 0;
   return per_cu * prop.multiProcessorCount;
 }
 
@@ -849,8 +865,11 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
   if (coop) {
     // A normal launch: hipLaunchCooperativeKernel adds about 11 us of idle
     // time on each side of the kernel, and the grid is sized to be resident.
-    k_rebuild_grid<H><<<P.rs_blocks, 256, 0, st>>>(P, B);
-    k_lists<H><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
+    const bool lng = P.nb_cap > PL_LDS_CAP, bvh = P.ntri > 0 && !P.wall_table;
+    if (!lng && !bvh) k_rebuild_grid<H, false, false><<<P.rs_blocks, 256, 0, st>>>(P, B);
+    else if (!lng) k_rebuild_grid<H, false, true><<<P.rs_blocks, 256, 0, st>>>(P, B);
+    else if (!bvh) k_rebuild_grid<H, true, false><<<P.rs_blocks, 256, 0, st>>>(P, B);
+    else k_rebuild_grid<H, true, true><<<P.rs_blocks, 256, 0, st>>>(P, B);
     return hipGetLastError();
   }
   const int g = blocks_for(P.N, 256);
@@ -858,8 +877,10 @@ hipError_t launch_rebuild(const SimParams& P, const Bufs<H>& B, bool coop, hipSt
   launch_radix_sort(P.N, P.rs_blocks, P.rs_chunk, B.key[0], B.idx[0], B.key[1], B.idx[1], B.inv, B.rs_hist,
                     B.ctl, true, st);
   k_permute<H><<<g, 256, 0, st>>>(P, B);
-  k_cells<H><<<g, 256, 0, st>>>(P, B);
-  k_lists<H><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
+  if (P.ntri > 0 && !P.wall_table) k_cells<H, true><<<g, 256, 0, st>>>(P, B);
+  else k_cells<H, false><<<g, 256, 0, st>>>(P, B);
+  if (P.nb_cap <= PL_LDS_CAP) k_lists<H, false><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
+  else k_lists<H, true><<<P.list_blocks, PL_BLOCK, 0, st>>>(P, B);
   return hipGetLastError();
 }
 

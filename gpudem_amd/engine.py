@@ -29,7 +29,7 @@ SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error"
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
            "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
            "gdem_batch_step", "gdem_batch_poll", "gdem_batch_download", "gdem_batch_scene_status",
-           "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing", "gdem_scene_set_history",
+           "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing", "gdem_batch_history", "gdem_batch_set_history", "gdem_scene_set_history",
            "gdem_settings_load", "gdem_settings_wall", "gdem_settings_particles", "gdem_output_init",
            "gdem_write_frame_bin", "gdem_write_pid_header", "gdem_write_pid_row", "gdem_write_initial_pos"]
 
@@ -108,6 +108,11 @@ class HistoryView(ctypes.Structure):
                 ("nhist", "hist_key", "hist_delta", "nwhist", "whist_key", "whist_delta")]
 
 
+class HistoryOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("nhist", "hist_key", "hist_delta", "nwhist", "whist_key", "whist_delta")]
+
+
 class Settings(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("input_mode", ctypes.c_int), ("inlet_box", ctypes.c_double * 6),
                 ("radius", ctypes.c_double), ("density", ctypes.c_double), ("cor", ctypes.c_double),
@@ -171,6 +176,8 @@ def lib():
         L.gdem_batch_info.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int), P(ctypes.c_int64),
                                       P(ctypes.c_int64)]
         L.gdem_batch_timing.argtypes = [vp, P(ctypes.c_double)]
+        L.gdem_batch_history.argtypes = [vp, ctypes.c_int, P(HistoryOut)]
+        L.gdem_batch_set_history.argtypes = [vp, ctypes.c_int, P(HistoryView)]
         L.gdem_scene_set_history.argtypes = [vp, P(HistoryView)]
         L.gdem_settings_load.argtypes = [ctypes.c_char_p, P(Settings)]
         L.gdem_settings_wall.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
@@ -506,6 +513,7 @@ class Batch:
                          graph_steps)
         self.h = ctypes.c_void_p()
         self.n_of = {}
+        self.caps = dict(hist=hist_cap or 16, whist=whist_cap or 8)
         ctx.check(lib().gdem_batch_create(ctx.h, ctypes.byref(bp), ctypes.byref(self.h)), "gdem_batch_create")
         ctx._scenes.add(self)
 
@@ -546,6 +554,36 @@ class Batch:
         if check:
             self.ctx.check(rc, "gdem_batch_scene_status")
         return {f: getattr(st, f) for f, _ in BatchSceneStats._fields_}
+
+    def history(self, sid):
+        """Contact histories of a scene in its particles' own ids (pair keys
+        partner ids, wall keys triangle ids of the scene's mesh)."""
+        n, c = self.n_of[sid], self.caps
+        d = dict(nhist=np.zeros(n, np.int32), hist_key=np.full((n, c["hist"]), -1, np.int32),
+                 hist_delta=np.zeros((n, c["hist"], 3)), nwhist=np.zeros(n, np.int32),
+                 whist_key=np.full((n, c["whist"]), -1, np.int32), whist_delta=np.zeros((n, c["whist"], 3)))
+        out = HistoryOut(*[_ptr(d[f]) for f, _ in HistoryOut._fields_])
+        self.ctx.check(lib().gdem_batch_history(self.h, int(sid), ctypes.byref(out)), "gdem_batch_history")
+        return d
+
+    def snapshot(self, sid):
+        """State and contact histories of a scene at the current batch step."""
+        d = self.download(sid)
+        d.update(self.history(sid))
+        return d
+
+    def set_history(self, sid, hist):
+        """Histories (as history() returns them) into a scene admitted and
+        not stepped yet."""
+        n, c = self.n_of[sid], self.caps
+        arrs = dict(nhist=np.ascontiguousarray(hist["nhist"], np.int32),
+                    hist_key=np.ascontiguousarray(hist["hist_key"], np.int32).reshape(n, c["hist"]),
+                    hist_delta=np.ascontiguousarray(hist["hist_delta"], np.float64).reshape(n, c["hist"], 3),
+                    nwhist=np.ascontiguousarray(hist["nwhist"], np.int32),
+                    whist_key=np.ascontiguousarray(hist["whist_key"], np.int32).reshape(n, c["whist"]),
+                    whist_delta=np.ascontiguousarray(hist["whist_delta"], np.float64).reshape(n, c["whist"], 3))
+        view = HistoryView(*[_ptr(arrs[f]) for f, _ in HistoryView._fields_])
+        self.ctx.check(lib().gdem_batch_set_history(self.h, int(sid), ctypes.byref(view)), "gdem_batch_set_history")
 
     def evict(self, sid):
         self.ctx.check(lib().gdem_batch_evict(self.h, int(sid)), "gdem_batch_evict")

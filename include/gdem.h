@@ -175,6 +175,18 @@ typedef struct {
   const double* whist_delta; /* [n*whist_cap*3] */
 } gdem_history_view;
 
+/* Output form of gdem_history_view: rows [n*hist_cap] / [n*whist_cap] in
+ * original particle order, pair keys as partner ids, wall keys as triangle
+ * ids, displacements [row*3]. */
+typedef struct {
+  int* nhist;
+  int* hist_key;
+  double* hist_delta;
+  int* nwhist;
+  int* whist_key;
+  double* whist_delta;
+} gdem_history_out;
+
 /* demSettings.json (main.cu:57-135, 227; ParticleSystem.cu:189-352). */
 #define GDEM_INLET_SHUFFLE 0
 #define GDEM_INLET_FILE 1
@@ -321,6 +333,15 @@ int gdem_batch_step(gdem_batch* batch, int nsteps);
 int gdem_batch_poll(gdem_batch* batch, int* scenes, int max_scenes, int* n_out);
 int gdem_batch_download(gdem_batch* batch, int scene, gdem_particles_out* out);
 int gdem_batch_scene_status(gdem_batch* batch, int scene, gdem_batch_scene_stats* stats);
+/* Snapshot and restore of a batch scene (the SAVE_ON members copyFromDevice
+ * and copyToDevice carry, ParticleSystem.cu:82-106), for evicting a running
+ * scene and resuming it later: gdem_batch_download gives its state and
+ * gdem_batch_history its contact histories at the current batch step;
+ * gdem_batch_admit from that state followed by gdem_batch_set_history,
+ * before the next gdem_batch_step (GDEM_E_STATE otherwise), resumes it.  A
+ * resumed scene continues bit for bit as if it had never left. */
+int gdem_batch_history(gdem_batch* batch, int scene, const gdem_history_out* out);
+int gdem_batch_set_history(gdem_batch* batch, int scene, const gdem_history_view* in);
 /* Frees a scene's slot and ranges (finished or not). */
 int gdem_batch_evict(gdem_batch* batch, int scene);
 /* Batch step count, live scenes, free particle slots, device bytes held. */

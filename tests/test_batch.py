@@ -202,3 +202,26 @@ def test_oversized_request_is_rejected(ctx):
     assert by[0].error is not None and by[0].result is None
     assert by[1].error is None and by[1].stats["steps_done"] == 200
     assert serve.summarize(done, 1.0)["failed"] == 1
+
+
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_preempted_request_resumes_bitwise(ctx, precision):
+    # One scene slot: a long request runs, a short one arrives and waits, so
+    # the server evicts the long one with a snapshot of its state and contact
+    # histories, serves the short one, then resumes the long one from the
+    # snapshot.  Both end exactly as when stepped alone.
+    from gpudem_amd import serve
+    long_req, short_req = mk(81, 2500, mu=0.6, cor=0.4), mk(82, 800)
+    stream = [(0.0, long_req, 3000), (0.02, short_req, 400)]
+    say(f"preemption, precision {precision}")
+    srv = serve.Server(ctx, precision, hbm_budget=64 << 20, max_scenes=1, chunk=100, graph_steps=50,
+                       preempt_after=0.0, preempt_min_steps=100, **CAPS)
+    done = srv.run(stream)
+    srv.close()
+    by = {r.rid: r for r in done}
+    assert len(done) == 2 and by[0].preemptions >= 1 and by[1].preemptions == 0
+    assert by[1].done < by[0].done
+    for rid, (_, spec, nsteps) in enumerate(stream):
+        req = by[rid]
+        assert req.error is None and req.stats["errors"] == 0 and req.stats["steps_done"] == nsteps
+        assert_same(req.result, alone(ctx, spec, precision, nsteps)[0], f"request {rid}")
