@@ -38,10 +38,11 @@ extern "C" {
 #define GDEM_ERR_WHIST_OVERFLOW 8u
 #define GDEM_ERR_VE_OVERFLOW 16u
 #define GDEM_ERR_NAN 32u
-#define GDEM_ERR_BARRIER 64u /* a grid barrier of the list rebuild timed out (GDEM_E_HIP from
-                                gdem_scene_status; the scene's lists are no longer rebuilt).  The
-                                environment variable GDEM_GBAR_SPIN, read at scene creation, lowers
-                                the barrier's poll limit to exercise this path. */
+#define GDEM_ERR_BARRIER 64u /* a grid barrier of the list rebuild timed out and was cancelled;
+                                the scene then rebuilds through per-phase kernels (reported once
+                                as GDEM_E_HIP by gdem_scene_status).  The environment variable
+                                GDEM_GBAR_SPIN, read at scene creation, lowers the barrier's poll
+                                limit to exercise this path. */
 
 typedef struct gdem_ctx gdem_ctx;
 typedef struct gdem_scene gdem_scene;
