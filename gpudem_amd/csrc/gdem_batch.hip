@@ -87,30 +87,40 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
   SlotCtl& C = B.sctl[sid];
   if (C.fl[rebuild_parity(step)] == 0) return;
   const BScene& S = B.scenes[sid];
-  __shared__ int part[256];
+  // per round: each thread sums SCAN_PER counts, each wave scans its 64 sums
+  // with lane shuffles, and the four wave totals are combined through LDS
+  // (two workgroup barriers per round)
+  __shared__ int part[4];
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int base = S.cell_base, n = S.ncell;
   int carry = 0;
+  // the next round's counts are requested before this round's stores, so
+  // their latency overlaps the scan
+  int v[SCAN_PER];
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) v[k] = (tid * SCAN_PER + k < n) ? B.ccount[base + tid * SCAN_PER + k] : 0;
   for (int c0 = 0; c0 < n; c0 += 256 * SCAN_PER) {
     const int beg = c0 + tid * SCAN_PER;
-    int v[SCAN_PER];
+    const int nbeg = beg + 256 * SCAN_PER;
+    int vn[SCAN_PER];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) vn[k] = (nbeg + k < n) ? B.ccount[base + nbeg + k] : 0;
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) {
-      v[k] = (beg + k < n) ? B.ccount[base + beg + k] : 0;
-      sum += v[k];
-    }
-    part[tid] = sum;
+    for (int k = 0; k < SCAN_PER; ++k) sum += v[k];
+    const int incl = wave_incl_sum(sum);
+    if (lane == 63) part[wave] = incl;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const int add = tid >= off ? part[tid - off] : 0;
-      __syncthreads();
-      part[tid] += add;
-      __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int t = part[w];
+      if (w < wave) before += t;
+      total += t;
     }
-    int run = S.seg0 + carry + part[tid] - sum;
-    const int total = part[255];
-    __syncthreads();
+    int run = S.seg0 + carry + before + incl - sum;
+    __syncthreads();  // every wave has read part[] before the next round writes it
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) {
       if (beg + k < n) {
@@ -120,6 +130,8 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
       }
     }
     carry += total;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
   }
   if (tid == 0) {
     C.nact = carry;
