@@ -1402,20 +1402,31 @@ __device__ __forceinline__ void pair_candidates(const ListArgs& L, int s, const 
       st[k] = ce.x;
       cnt[k] = ce.x < 0 ? 0 : ce.y - ce.x;
     }
+    // The slab's t-th particle is slot t + off(k) for the last cell k whose
+    // first particle is number pre[k] <= t (off(k) = its first slot minus
+    // pre[k]); off is formed as a telescoping sum of the differences dlt, so
+    // no cell is indexed at run time (empty cells cancel out).
+    int pre[9], dlt[9];
+    {
+      int prev = 0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) total += cnt[k];
+      for (int k = 0; k < 9; ++k) {
+        const int off = st[k] - total;
+        pre[k] = total;
+        dlt[k] = off - prev;
+        prev = off;
+        total += cnt[k];
+      }
+    }
     for (int t0 = 0; t0 < total; t0 += PL_BATCH) {
       int qv[PL_BATCH];
 #pragma unroll
       for (int u = 0; u < PL_BATCH; ++u) {
-        // slot of the slab's (t0 + u)-th particle
-        int rem = t0 + u, q = -1;
+        const int t = t0 + u;
+        int q = t;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          if (q < 0 && rem < cnt[k]) q = st[k] + rem;
-          if (q < 0) rem -= cnt[k];
-        }
-        qv[u] = (t0 + u < total && q != s) ? q : -1;
+        for (int k = 0; k < 9; ++k) q += t >= pre[k] ? dlt[k] : 0;
+        qv[u] = (t < total && q != s) ? q : -1;
       }
       double4 pj[PL_BATCH];
       int idj[PL_BATCH];
