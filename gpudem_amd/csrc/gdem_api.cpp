@@ -45,11 +45,15 @@ struct gdem_scene {
   uint32_t* h_abort = nullptr;        // pinned copy of the barrier's abort word
   hipEvent_t abort_ev = nullptr;      // recorded after that copy
   bool abort_probe = false;           // a copy is in flight or done
+  int* h_lstat = nullptr;             // pinned copy of the control words from CTL_NACT (active count,
+                                      // and at +3 the sum of the pair-list lengths of the last rebuild)
+  bool lstat_probe = false;           // a copy is in flight or done (abort_ev follows it)
 
   ~gdem_scene() {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (abort_ev) (void)hipEventDestroy(abort_ev);
     if (h_abort) (void)hipHostFree(h_abort);
+    if (h_lstat) (void)hipHostFree(h_lstat);
     for (auto e : ev) (void)hipEventDestroy(e);
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
@@ -162,16 +166,42 @@ int recover_barrier(gdem_scene* sc, bool wait) {
   return GDEM_OK;
 }
 
-// After the steps of a call, the abort word is copied to pinned host memory
-// behind them, so the next call can see a cancelled barrier without waiting.
+// After the steps of a call, the abort word and the list statistics are
+// copied to pinned host memory behind them, so the next call can see a
+// cancelled barrier and the current mean list length without waiting.
 template <class H>
 int probe_barrier(gdem_scene* sc) {
-  if (!sc->coop) return GDEM_OK;
   gdem_ctx* c = sc->ctx;
-  HIPCHK(c, hipMemcpyAsync(sc->h_abort, bufs<H>(sc).gbar + GBAR_ABORT, 4, hipMemcpyDeviceToHost, c->stream));
+  if (sc->coop)
+    HIPCHK(c, hipMemcpyAsync(sc->h_abort, bufs<H>(sc).gbar + GBAR_ABORT, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sc->h_lstat, sc->d_ctl + CTL_NACT, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipEventRecord(sc->abort_ev, c->stream));
-  sc->abort_probe = true;
+  sc->abort_probe = sc->coop;
+  sc->lstat_probe = true;
   return GDEM_OK;
+}
+
+// The step fetches its first 16 pair-list entries together with the own
+// records (one memory round trip fewer) when the lists are long enough to use
+// them: at a mean length of LIST_PREFETCH_MIN or more.  Shorter lists would
+// mostly fetch rows past their ends, so they are read on demand.  The choice
+// follows the lengths the last rebuild built, as seen by the latest probe; a
+// change drops the captured graph, which holds the other kernel.
+constexpr int LIST_PREFETCH_MIN = 4;
+
+void set_prefetch(gdem_scene* sc, long long nbsum, long long nact) {
+  const int want = nbsum >= (long long)LIST_PREFETCH_MIN * nact && nact > 0 ? 1 : 0;
+  if (want == sc->P.list_prefetch) return;
+  sc->P.list_prefetch = want;
+  if (sc->graph) {
+    (void)hipGraphExecDestroy(sc->graph);
+    sc->graph = nullptr;
+  }
+}
+
+void choose_prefetch(gdem_scene* sc) {
+  if (!sc->lstat_probe || hipEventQuery(sc->abort_ev) != hipSuccess) return;
+  set_prefetch(sc, sc->h_lstat[3], sc->h_lstat[0]);
 }
 
 template <class H>
@@ -183,6 +213,7 @@ int step_impl(gdem_scene* sc, int nsteps) {
     const int rc = recover_barrier<H>(sc, false);
     if (rc) return rc;
   }
+  choose_prefetch(sc);
   if (sc->timing > 0) {
     // direct launches; an event pair brackets the force kernel of every
     // `timing`-th step (sampling keeps the host ahead of the device)
@@ -303,9 +334,11 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   HIPCHK(c, hipStreamSynchronize(st));
   rc = recover_barrier<H>(sc, true);
   if (rc) return rc;
-  int rb = 0;
-  HIPCHK(c, hipMemcpy(&rb, sc->d_ctl + CTL_REBUILDS, sizeof(int), hipMemcpyDeviceToHost));
-  sc->rebuild_base = rb;
+  int ctl2[CTL_WORDS];
+  HIPCHK(c, hipMemcpy(ctl2, sc->d_ctl, sizeof(ctl2), hipMemcpyDeviceToHost));
+  sc->rebuild_base = ctl2[CTL_REBUILDS];
+  sc->P.list_prefetch = 1;
+  set_prefetch(sc, ctl2[CTL_NBSUM], ctl2[CTL_NACT]);
   return GDEM_OK;
 }
 
@@ -728,6 +761,8 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
   HIPCHK(c, hipEventCreateWithFlags(&sc->abort_ev, hipEventDisableTiming));
   HIPCHK(c, hipHostMalloc((void**)&sc->h_abort, sizeof(uint32_t), hipHostMallocDefault));
   *sc->h_abort = 0;
+  HIPCHK(c, hipHostMalloc((void**)&sc->h_lstat, 4 * sizeof(int), hipHostMallocDefault));
+  for (int k = 0; k < 4; ++k) sc->h_lstat[k] = 0;
   int rc = sc->precision == GDEM_FP64 ? create_impl<double>(sc.get(), pd) : create_impl<float>(sc.get(), pd);
   if (rc) return rc;
   *out = sc.release();

@@ -1461,7 +1461,7 @@ struct PairLdsT {
 using PairLds = PairLdsT<PL_BLOCK>;
 
 template <int COLS>
-__device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col) {
+__device__ __forceinline__ int pairlist_lds(const ListArgs& L, int s, int2* col) {
   const int N = L.N;
   const double4 p = L.pos[s];
   const int cap = L.nb_cap;
@@ -1489,9 +1489,10 @@ __device__ __forceinline__ void pairlist_lds(const ListArgs& L, int s, int2* col
   for (int k = 0; k < n; ++k) L.nbr[(size_t)k * N + s] = col[k * COLS].x;
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
+  return n;
 }
 
-__device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
+__device__ __forceinline__ int pairlist_arrays(const ListArgs& L, int s) {
   const int N = L.N;
   const double4 p = L.pos[s];
   const int cap = L.nb_cap < NB_MAX ? L.nb_cap : NB_MAX;
@@ -1517,13 +1518,15 @@ __device__ __forceinline__ void pairlist_arrays(const ListArgs& L, int s) {
   for (int q = 0; q < n; ++q) L.nbr[(size_t)q * N + s] = cand[q];
   L.nnb[s] = n;
   if (ovf) atomicOr(L.err, GDEM_ERR_NB_OVERFLOW_BIT);
+  return n;
 }
 
 // lds: the workgroup's list columns (one per thread of a COLS-thread
 // workgroup).  LONG: capacities above PL_LDS_CAP, gathered in private arrays
 // (a separate kernel instantiation, so the usual one carries no scratch).
+// Returns the list length.
 template <int COLS, bool LONG>
-__device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool active, PairLdsT<COLS>& lds) {
+__device__ __forceinline__ int pairlist_one(const ListArgs& L, int s, bool active, PairLdsT<COLS>& lds) {
   {
     const double4 p = L.pos[s];
     L.ref[s] = p;
@@ -1531,10 +1534,10 @@ __device__ __forceinline__ void pairlist_one(const ListArgs& L, int s, bool acti
   }
   if (!active) {
     L.nnb[s] = 0;
-    return;
+    return 0;
   }
-  if constexpr (LONG) pairlist_arrays(L, s);
-  else pairlist_lds<COLS>(L, s, &lds.e[0][threadIdx.x]);
+  if constexpr (LONG) return pairlist_arrays(L, s);
+  else return pairlist_lds<COLS>(L, s, &lds.e[0][threadIdx.x]);
 }
 
 // Wall candidates by BVH traversal (k_update_neighborlist_wall,

@@ -17,7 +17,7 @@ namespace gdem {
 // ---------------------------------------------------------------------------
 // fused contact forces + integration for one step
 // ---------------------------------------------------------------------------
-template <class T, class H, bool REC, class PRow, class WRow>
+template <class T, class H, bool REC, bool PRE, class PRow, class WRow>
 __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
   __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -60,12 +60,12 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   // this workgroup's positions, for in-tile neighbour reads
   __shared__ double4 tile_p[256];
   const bool valid = s < P.N;
-  const Own<H> own = load_own<true>(A, s, valid);
+  const Own<H> own = load_own<PRE>(A, s, valid);
   tile_p[threadIdx.x] = own.p;
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
-  step_particle<T, H, REC, true, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
+  step_particle<T, H, REC, PRE, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
   if (err) atomicOr((unsigned int*)&B.ctl[CTL_ERR], err);
   if (last_block(&B.ctl[CTL_DONE_STEP])) {
@@ -331,6 +331,7 @@ __global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < P.N) keys_one(P, B, s, B.ctl[CTL_CUR]);
+  if (s == 0) B.ctl[CTL_NBSUM] = 0;
 }
 
 template <bool GATED>
@@ -434,6 +435,15 @@ __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
   }
 }
 
+// Adds a thread's list lengths to the workgroup's sum in LDS (zeroed before
+// a barrier that precedes this call) and the workgroup's sum to the scene's
+// control word; every thread of the workgroup calls it.
+__device__ __forceinline__ void add_list_lengths(int* ctl_sum, int* wg_sum, int mine) {
+  if (mine) atomicAdd(wg_sum, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && *wg_sum) atomicAdd(ctl_sum, *wg_sum);
+}
+
 // Pair lists of the sorted particles, at full occupancy (both the chain and
 // the grid-barrier rebuild end here).  The last workgroup clears the refresh
 // flag, which ends the rebuild.
@@ -444,10 +454,15 @@ __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   // place for the host to rebuild through the launch chain
   if (__builtin_amdgcn_readfirstlane(B.gbar[GBAR_ABORT]) != 0) return;
   __shared__ PairLds lds;
+  __shared__ int nbsum;
+  if (threadIdx.x == 0) nbsum = 0;
+  __syncthreads();
   const ListArgs L = list_args(P, B, B.ctl[CTL_CUR], B.ctl[CTL_HCUR]);
   const int nact = B.ctl[CTL_NACT];
+  int mine = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.N; s += gridDim.x * blockDim.x)
-    pairlist_one<PL_BLOCK, LONG>(L, s, s < nact, lds);
+    mine += pairlist_one<PL_BLOCK, LONG>(L, s, s < nact, lds);
+  add_list_lengths(&B.ctl[CTL_NBSUM], &nbsum, mine);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -564,6 +579,11 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   } lds;
   uint32_t* dig = lds.rs.dig;
   uint16_t* wcnt = lds.rs.wcnt;
+  __shared__ int nbsum;
+  if (threadIdx.x == 0) {
+    nbsum = 0;
+    if (b == 0) B.ctl[CTL_NBSUM] = 0;
+  }
   // keys of this workgroup's chunk, counted for the first pass as they are made
   rs_clear(dig);
   {
@@ -598,7 +618,9 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   // pair lists: every slot's neighbour cells and records are in place
   if (!grid_barrier(B.gbar, B.ctl, P.gbar_spin)) return;
   const int nact = B.ctl[CTL_NACT];
-  for (int s = first; s < N; s += stride) pairlist_one<256, LONG>(L, s, s < nact, lds.pl);
+  int mine = 0;
+  for (int s = first; s < N; s += stride) mine += pairlist_one<256, LONG>(L, s, s < nact, lds.pl);
+  add_list_lengths(&B.ctl[CTL_NBSUM], &nbsum, mine);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -809,16 +831,21 @@ template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
   Rec<T> none{};
   const int g = blocks_for(P.N, 256);
-  if (P.hist_cap <= PCAP_REG && P.whist_cap <= WCAP_REG)
-    k_step<T, H, false, HistLds<H, PCAP_REG>, HistLds<H, WCAP_REG>><<<g, 256, 0, st>>>(P, B, none);
+  const bool lds_rows = P.hist_cap <= PCAP_REG && P.whist_cap <= WCAP_REG;
+  if (lds_rows && P.list_prefetch)
+    k_step<T, H, false, true, HistLds<H, PCAP_REG>, HistLds<H, WCAP_REG>><<<g, 256, 0, st>>>(P, B, none);
+  else if (lds_rows)
+    k_step<T, H, false, false, HistLds<H, PCAP_REG>, HistLds<H, WCAP_REG>><<<g, 256, 0, st>>>(P, B, none);
+  else if (P.list_prefetch)
+    k_step<T, H, false, true, HistScan<H>, HistScan<H>><<<g, 256, 0, st>>>(P, B, none);
   else
-    k_step<T, H, false, HistScan<H>, HistScan<H>><<<g, 256, 0, st>>>(P, B, none);
+    k_step<T, H, false, false, HistScan<H>, HistScan<H>><<<g, 256, 0, st>>>(P, B, none);
 }
 
 template <class T, class H>
 void launch_step_record(const SimParams& P, const Bufs<H>& B, const RecDev& r, hipStream_t st) {
   Rec<T> rec{r.cap, r.count, r.key, r.kind, r.fn, r.ft, r.delta, r.acc, r.anga};
-  k_step<T, H, true, HistScan<H>, HistScan<H>><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
+  k_step<T, H, true, true, HistScan<H>, HistScan<H>><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, rec);
 }
 
 void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1, int* inv_out,

@@ -49,6 +49,7 @@ struct SimParams {
   int list_blocks;          // grid of the pair-list kernel (co-resident workgroups, grid-stride)
   int gbar_spin;            // poll limit of the rebuild kernel's grid barriers
   int wall_table;           // 1: wall candidates from the static per-cell table, 0: BVH traversal
+  int list_prefetch;        // 1: the step fetches the first 16 pair-list entries with the own records
 };
 
 // Control words shared by all kernels of a scene (device memory).
@@ -62,6 +63,7 @@ enum Ctl : int {
   CTL_NACT = 6,      // active particles after the last re-sort
   CTL_DONE_WALL = 7,
   CTL_REBUILDS = 8,
+  CTL_NBSUM = 9,     // sum of the pair-list lengths built by the last rebuild
   CTL_WORDS = 16
 };
 
