@@ -73,17 +73,29 @@ __global__ __launch_bounds__(256) void kb_bin(BBufs<H> B) {
     B.bin_cell[i] = -1;
     B.bin_rank[i] = atomicAdd(&C.inact, 1);
   }
+  // this wave's particles per chunk of the scene's cells (each workgroup of
+  // the scan starts from the particles of the chunks before its own)
+  const int cc = (S.ncell + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+  const int ch = in ? (c - S.cell_base) / cc : -1;
+#pragma unroll
+  for (int k = 0; k < SCAN_CHUNKS; ++k) {
+    const int m = __popcll(__ballot(ch == k));
+    if (m && __lane_id() == 0) atomicAdd(&C.chunk[k], m);
+  }
 }
 
-// One workgroup per due scene: exclusive scan of the scene's cell counts into
-// the (first, end) slot table, clearing the counters for the next rebuild.
+// SCAN_CHUNKS workgroups per due scene, each a contiguous chunk of its cells:
+// exclusive scan of the cell counts into the (first, end) slot table,
+// starting from the particles the binning pass counted in the chunks before
+// (no workgroup waits for another), clearing the counters for the next
+// rebuild.  The last chunk's workgroup records the scene's active count.
 constexpr int SCAN_PER = 16;
 
 template <class H>
 __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
   const int step = B.ctl[BC_STEP];
   if (B.ctl[BC_ANY0 + rebuild_parity(step)] == 0) return;
-  const int sid = blockIdx.x;
+  const int sid = blockIdx.x / SCAN_CHUNKS, part_k = blockIdx.x % SCAN_CHUNKS;
   SlotCtl& C = B.sctl[sid];
   if (C.fl[rebuild_parity(step)] == 0) return;
   const BScene& S = B.scenes[sid];
@@ -94,18 +106,24 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int base = S.cell_base, n = S.ncell;
+  const int cc = (n + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+  const int lo = part_k * cc, hi = min(n, lo + cc);
   int carry = 0;
+  for (int k = 0; k < part_k; ++k) carry += C.chunk[k];
   // the next round's counts are requested before this round's stores, so
   // their latency overlaps the scan
   int v[SCAN_PER];
 #pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) v[k] = (tid * SCAN_PER + k < n) ? B.ccount[base + tid * SCAN_PER + k] : 0;
-  for (int c0 = 0; c0 < n; c0 += 256 * SCAN_PER) {
+  for (int k = 0; k < SCAN_PER; ++k) {
+    const int x = lo + tid * SCAN_PER + k;
+    v[k] = x < hi ? B.ccount[base + x] : 0;
+  }
+  for (int c0 = lo; c0 < hi; c0 += 256 * SCAN_PER) {
     const int beg = c0 + tid * SCAN_PER;
     const int nbeg = beg + 256 * SCAN_PER;
     int vn[SCAN_PER];
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) vn[k] = (nbeg + k < n) ? B.ccount[base + nbeg + k] : 0;
+    for (int k = 0; k < SCAN_PER; ++k) vn[k] = (nbeg + k < hi) ? B.ccount[base + nbeg + k] : 0;
     int sum = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) sum += v[k];
@@ -123,7 +141,7 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
     __syncthreads();  // every wave has read part[] before the next round writes it
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) {
-      if (beg + k < n) {
+      if (beg + k < hi) {
         B.cell_tab[base + beg + k] = make_int2(run, run + v[k]);
         if (v[k]) B.ccount[base + beg + k] = 0;
         run += v[k];
@@ -133,7 +151,7 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
   }
-  if (tid == 0) {
+  if (tid == 0 && part_k == SCAN_CHUNKS - 1) {
     C.nact = carry;
     C.inact = 0;
     C.q2 = C.q ^ 1;
@@ -234,6 +252,8 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   L.ntri = S.ntri;
   L.wall_table = 1;
   const bool active = (i - S.seg0) < C.nact;
+  if (i == S.seg0)
+    for (int k = 0; k < SCAN_CHUNKS; ++k) C.chunk[k] = 0;  // read by this rebuild's scan only
   pairlist_one<PL_BLOCK, LONG>(L, i, active, lds);
   walllist_one<false>(L, i, active);  // batch scenes always have the per-cell table
   // every later reader of q runs in a later kernel; this kernel reads q2 only
@@ -383,7 +403,7 @@ void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st) {
   if (grid <= 0) return;
   const int nslots = B.nslots;
   kb_bin<H><<<grid, 256, 0, st>>>(B);
-  kb_scan<H><<<nslots, 256, 0, st>>>(B);
+  kb_scan<H><<<nslots * SCAN_CHUNKS, 256, 0, st>>>(B);
   kb_permute<H><<<grid, 256, 0, st>>>(B);
   if (B.nb_cap <= PL_LDS_CAP) kb_lists<H, false><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
   else kb_lists<H, true><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);

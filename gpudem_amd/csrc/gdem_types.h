@@ -150,6 +150,10 @@ struct BScene {
 // (STEP & 1) ^ q and its current material/history buffer is q: a rebuild
 // permutes the scene into the other buffers and flips q, leaving every other
 // scene untouched.
+// A scene's cell table is scanned by SCAN_CHUNKS workgroups, one per
+// contiguous chunk of its cells.
+constexpr int SCAN_CHUNKS = 8;
+
 struct SlotCtl {
   int q;        // buffer parity
   int q2;       // parity after the rebuild in progress
@@ -158,6 +162,7 @@ struct SlotCtl {
   int nact;     // active particles after the last rebuild
   unsigned int err;
   int rebuilds;
+  int chunk[SCAN_CHUNKS];  // active particles binned into each chunk of the cells (zero outside a rebuild)
 };
 
 enum BCtl : int {
