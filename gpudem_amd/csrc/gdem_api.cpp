@@ -364,6 +364,30 @@ int download_impl(gdem_scene* sc, gdem_particles_out* out) {
 }
 
 template <class H>
+int frame_impl(gdem_scene* sc, double lf, float* xyz_out, float* r_out, int* n_active) {
+  gdem_ctx* c = sc->ctx;
+  const size_t N = (size_t)sc->P.N;
+  DevArena tmp;
+  float *xyz, *r;
+  int* cnt;
+  const int nb = frame_blocks((int)N);
+  HIPCHK(c, tmp.alloc(&xyz, 3 * N));
+  HIPCHK(c, tmp.alloc(&r, N));
+  HIPCHK(c, tmp.alloc(&cnt, (size_t)nb + 1));
+  launch_frame<H>(sc->P, bufs<H>(sc), cnt, lf, xyz, r, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int na = 0;
+  HIPCHK(c, hipMemcpy(&na, cnt + nb, sizeof(int), hipMemcpyDeviceToHost));
+  if (na > 0) {
+    HIPCHK(c, hipMemcpy(xyz_out, xyz, (size_t)na * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(r_out, r, (size_t)na * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  *n_active = na;
+  return GDEM_OK;
+}
+
+template <class H>
 int debug_impl(gdem_scene* sc, gdem_debug_view* vw) {
   gdem_ctx* c = sc->ctx;
   const size_t N = (size_t)sc->P.N;
@@ -796,6 +820,12 @@ int gdem_scene_status(gdem_scene* sc, gdem_stats* st) {
                 "list rebuild: a grid barrier timed out; the scene now rebuilds through per-phase kernels");
   }
   return GDEM_OK;
+}
+
+int gdem_scene_frame(gdem_scene* sc, double length_factor, float* xyz_out, float* r_out, int* n_active) {
+  if (!sc || !xyz_out || !r_out || !n_active) return GDEM_E_ARG;
+  return sc->precision == GDEM_FP64 ? frame_impl<double>(sc, length_factor, xyz_out, r_out, n_active)
+                                    : frame_impl<float>(sc, length_factor, xyz_out, r_out, n_active);
 }
 
 int gdem_scene_download_noalloc(gdem_scene* sc, gdem_particles_out* out) {

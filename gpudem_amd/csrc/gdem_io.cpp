@@ -53,6 +53,24 @@ int gdem_write_frame_bin(const char* dir, int step, int n, const double* x, cons
   return ok ? GDEM_OK : GDEM_E_ARG;
 }
 
+// The same file from the compacted arrays gdem_scene_frame produces on the
+// device (n_active float32 xyz triples and radii, already scaled).
+int gdem_write_frame_compact(const char* dir, int step, int n_active, const float* xyz, const float* r) {
+  if (!dir || n_active < 0 || (n_active > 0 && (!xyz || !r))) return GDEM_E_ARG;
+  char name[4096];
+  std::snprintf(name, sizeof(name), "%s/result_%012d.bin", dir, step);
+  FILE* fp = std::fopen(name, "wb");
+  if (!fp) return GDEM_E_ARG;
+  const size_t n = (size_t)n_active;
+  bool ok = std::fwrite(&n_active, sizeof(int), 1, fp) == 1;
+  if (n) {
+    ok = ok && std::fwrite(xyz, sizeof(float), 3 * n, fp) == 3 * n;
+    ok = ok && std::fwrite(r, sizeof(float), n, fp) == n;
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  return ok ? GDEM_OK : GDEM_E_ARG;
+}
+
 // write_header_text (solver_output.cu:62-85): dir/pid<pid>.txt created with
 // its column header.
 int gdem_write_pid_header(const char* dir, int pid) {

@@ -119,6 +119,12 @@ template <class H>
 void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc, hipStream_t st);
 template <class H>
 void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st);
+// The active particles' positions and radii in original id order, scaled by
+// lf and converted to float32 (cnt: frame_blocks(N) + 1 ints of workspace;
+// cnt[frame_blocks(N)] receives the active count).  Uses inv.
+template <class H>
+void launch_frame(const SimParams& P, const Bufs<H>& B, int* cnt, double lf, float* xyz, float* r, hipStream_t st);
+int frame_blocks(int n);
 
 void launch_radix_sort(int n, int blocks, int chunk, uint32_t* k0, int* i0, uint32_t* k1, int* i1, int* inv_out,
                        uint32_t* hist, const int* ctl, bool gated, hipStream_t st);

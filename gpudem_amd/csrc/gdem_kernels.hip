@@ -717,6 +717,93 @@ __global__ __launch_bounds__(256) void k_slot_of_id(SimParams P, Bufs<H> B) {
   if ((unsigned)id < (unsigned)P.N) B.inv[id] = s;
 }
 
+// ---- solver_output frames compacted on the device ---------------------------
+// A frame holds the active particles in original id order (write_frame_bin,
+// solver_output.cu:259-316).  inv maps each id to its slot (k_slot_of_id);
+// a workgroup takes FRAME_BLOCK consecutive ids, FRAME_PER per thread.
+constexpr int FRAME_PER = 8;
+constexpr int FRAME_BLOCK = 256 * FRAME_PER;
+
+// Exclusive scan over the workgroup of one count per thread (4 waves, LDS
+// for the wave totals); returns the total in *total.
+__device__ __forceinline__ int wg_excl_sum(int v, int* part, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_incl_sum(v);
+  if (lane == 63) part[wave] = incl;
+  __syncthreads();
+  int before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int t = part[w];
+    if (w < wave) before += t;
+    all += t;
+  }
+  *total = all;
+  return before + incl - v;
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void k_frame_count(SimParams P, Bufs<H> B, int* cnt) {
+  __shared__ int part[4];
+  const int cur = B.ctl[CTL_CUR];
+  const int id0 = blockIdx.x * FRAME_BLOCK + threadIdx.x * FRAME_PER;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < FRAME_PER; ++k) {
+    const int id = id0 + k;
+    if (id < P.N) c += B.pos[cur][B.inv[id]].w >= 0. ? 1 : 0;
+  }
+  int total = 0;
+  (void)wg_excl_sum(c, part, &total);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of the nb block counts in place, the sum in
+// cnt[nb].
+__global__ __launch_bounds__(256) void k_frame_scan(int nb, int* cnt) {
+  __shared__ int part[4];
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += 256) {
+    const int b = b0 + (int)threadIdx.x;
+    const int v = b < nb ? cnt[b] : 0;
+    int total = 0;
+    const int ex = wg_excl_sum(v, part, &total);
+    __syncthreads();  // every wave has read part[] before the next round writes it
+    if (b < nb) cnt[b] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) cnt[nb] = carry;
+}
+
+// The frame's float32 values, each scaled by lf in FP64 before the
+// conversion, at the active particles' ranks.
+template <class H>
+__global__ __launch_bounds__(256) void k_frame_write(SimParams P, Bufs<H> B, const int* off, double lf, float* xyz,
+                                                     float* r) {
+  __shared__ int part[4];
+  const int cur = B.ctl[CTL_CUR];
+  const int id0 = blockIdx.x * FRAME_BLOCK + threadIdx.x * FRAME_PER;
+  double4 p[FRAME_PER];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < FRAME_PER; ++k) {
+    const int id = id0 + k;
+    p[k] = id < P.N ? B.pos[cur][B.inv[id]] : make_double4(0., 0., 0., -1.);
+    c += p[k].w >= 0. ? 1 : 0;
+  }
+  int total = 0;
+  int at = off[blockIdx.x] + wg_excl_sum(c, part, &total);
+#pragma unroll
+  for (int k = 0; k < FRAME_PER; ++k) {
+    if (p[k].w < 0.) continue;
+    xyz[(size_t)at * 3 + 0] = (float)(p[k].x * lf);
+    xyz[(size_t)at * 3 + 1] = (float)(p[k].y * lf);
+    xyz[(size_t)at * 3 + 2] = (float)(p[k].z * lf);
+    r[at] = (float)(p[k].w * lf);
+    ++at;
+  }
+}
+
 // Contact histories from original particle order into the current internal
 // order (snapshot restore); pair keys become partner slots.  Rows longer than the capacity raise the
 // overflow bits and are cut at the capacity.
@@ -933,6 +1020,17 @@ void launch_counts(const SimParams& P, const Bufs<H>& B, unsigned long long* acc
 }
 
 template <class H>
+void launch_frame(const SimParams& P, const Bufs<H>& B, int* cnt, double lf, float* xyz, float* r, hipStream_t st) {
+  const int nb = blocks_for(P.N, FRAME_BLOCK);
+  k_slot_of_id<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B);
+  k_frame_count<H><<<nb, 256, 0, st>>>(P, B, cnt);
+  k_frame_scan<<<1, 256, 0, st>>>(nb, cnt);
+  k_frame_write<H><<<nb, 256, 0, st>>>(P, B, cnt, lf, xyz, r);
+}
+
+int frame_blocks(int n) { return blocks_for(n, FRAME_BLOCK); }
+
+template <class H>
 void launch_set_history(const SimParams& P, const Bufs<H>& B, const HistIn& in, hipStream_t st) {
   k_slot_of_id<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B);
   k_set_history<H><<<blocks_for(P.N, 256), 256, 0, st>>>(P, B, in);
@@ -966,6 +1064,8 @@ template void launch_debug_order<double>(const SimParams&, const Bufs<double>&, 
 template void launch_debug_order<float>(const SimParams&, const Bufs<float>&, const OrderDev&, hipStream_t);
 template void launch_counts<double>(const SimParams&, const Bufs<double>&, unsigned long long*, hipStream_t);
 template void launch_counts<float>(const SimParams&, const Bufs<float>&, unsigned long long*, hipStream_t);
+template void launch_frame<double>(const SimParams&, const Bufs<double>&, int*, double, float*, float*, hipStream_t);
+template void launch_frame<float>(const SimParams&, const Bufs<float>&, int*, double, float*, float*, hipStream_t);
 template void launch_set_history<double>(const SimParams&, const Bufs<double>&, const HistIn&, hipStream_t);
 template void launch_set_history<float>(const SimParams&, const Bufs<float>&, const HistIn&, hipStream_t);
 

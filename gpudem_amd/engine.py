@@ -24,14 +24,16 @@ ERRORS = {-1: "GDEM_E_ARG", -2: "GDEM_E_HIP", -3: "GDEM_E_OOM", -4: "GDEM_E_SLOT
 # exported symbols (checked by the CPU test suite against include/gdem.h)
 SYMBOLS = ["gdem_ctx_create", "gdem_ctx_destroy", "gdem_sync", "gdem_last_error", "gdem_device_arch", "gdem_material",
            "gdem_bbox_rule", "gdem_params_default", "gdem_scene_create", "gdem_scene_step",
-           "gdem_scene_status", "gdem_scene_download", "gdem_scene_download_noalloc", "gdem_scene_destroy", "gdem_scene_debug",
+           "gdem_scene_status", "gdem_scene_download", "gdem_scene_download_noalloc", "gdem_scene_frame", "gdem_scene_destroy",
+           "gdem_scene_debug",
            "gdem_scene_step_record", "gdem_scene_debug_order", "gdem_scene_timing", "gdem_scene_set_timing",
            "gdem_radix_sort", "gdem_tri_distance", "gdem_fp64_probe", "gdem_mesh_precompute",
            "gdem_copy_probe", "gdem_batch_create", "gdem_batch_destroy", "gdem_batch_admit",
            "gdem_batch_step", "gdem_batch_poll", "gdem_batch_download", "gdem_batch_scene_status",
            "gdem_batch_evict", "gdem_batch_info", "gdem_batch_timing", "gdem_batch_history", "gdem_batch_set_history", "gdem_scene_set_history",
            "gdem_settings_load", "gdem_settings_wall", "gdem_settings_particles", "gdem_output_init",
-           "gdem_write_frame_bin", "gdem_write_pid_header", "gdem_write_pid_row", "gdem_write_initial_pos"]
+           "gdem_write_frame_bin", "gdem_write_frame_compact", "gdem_write_pid_header", "gdem_write_pid_row",
+           "gdem_write_initial_pos"]
 
 
 class GdemError(RuntimeError):
@@ -151,6 +153,7 @@ def lib():
         L.gdem_scene_status.argtypes = [vp, P(Stats)]
         L.gdem_scene_download.argtypes = [vp, P(ParticlesOut)]
         L.gdem_scene_download_noalloc.argtypes = [vp, P(ParticlesOut)]
+        L.gdem_scene_frame.argtypes = [vp, ctypes.c_double, vp, vp, P(ctypes.c_int)]
         L.gdem_scene_destroy.argtypes = [vp]
         L.gdem_scene_destroy.restype = None
         L.gdem_scene_debug.argtypes = [vp, P(DebugView)]
@@ -185,6 +188,7 @@ def lib():
         L.gdem_output_init.argtypes = [ctypes.c_char_p]
         L.gdem_write_frame_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                            ctypes.c_double]
+        L.gdem_write_frame_compact.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, vp, vp]
         L.gdem_write_pid_header.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.gdem_write_pid_row.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, vp, vp,
                                          vp, vp, vp, ctypes.c_int]
@@ -408,6 +412,17 @@ class Scene:
         po = ParticlesOut(_ptr(out["x"]), _ptr(out["v"]), _ptr(out["angv"]), _ptr(out["active"]))
         self.ctx.check(lib().gdem_scene_download_noalloc(self.h, ctypes.byref(po)), "gdem_scene_download_noalloc")
         return out
+
+    def frame(self, length_factor=1.0):
+        """The active particles' positions (float32, n_active x 3) and radii
+        (float32) in original id order, scaled by length_factor, compacted on
+        the device (the data of one result_%012d.bin frame)."""
+        xyz = np.empty((self.n, 3), np.float32)
+        r = np.empty(self.n, np.float32)
+        na = ctypes.c_int(0)
+        self.ctx.check(lib().gdem_scene_frame(self.h, float(length_factor), _ptr(xyz), _ptr(r), ctypes.byref(na)),
+                       "gdem_scene_frame")
+        return xyz[:na.value], r[:na.value]
 
     def debug(self):
         n, c = self.n, self.caps
@@ -665,6 +680,17 @@ def write_frame_bin(d, step, x, r, active, length_factor=1.0):
     rc = lib().gdem_write_frame_bin(_b(d), int(step), len(r), _ptr(x), _ptr(r), _ptr(a), float(length_factor))
     if rc:
         raise GdemError(rc, "gdem_write_frame_bin")
+
+
+def write_frame_compact(d, step, xyz, r):
+    """result_%012d.bin from Scene.frame()'s arrays."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    r = np.ascontiguousarray(r, np.float32).ravel()
+    if len(r) != len(xyz):
+        raise ValueError("write_frame_compact: xyz and r differ in length")
+    rc = lib().gdem_write_frame_compact(_b(d), int(step), len(r), _ptr(xyz), _ptr(r))
+    if rc:
+        raise GdemError(rc, "gdem_write_frame_compact")
 
 
 def write_pid_header(d, pid):

@@ -39,11 +39,13 @@ def run_case(ctx, settings_path, outdir, precision=GDEM_FP64, steps=None, num_wr
             sc.step(out_step - 1)
         rec = sc.step_record(cap=1)
         done += out_step
-        d = sc.download()
-        engine.write_frame_bin(outdir, done, d["x"], spec.r, d["active"])
-        for pid in range(min(num_write, spec.n)):
-            engine.write_pid_row(outdir, done, st["dt"], pid, d["x"][pid], d["v"][pid], rec["accel"][pid],
-                                 d["angv"][pid], rec["ang_accel"][pid], d["active"][pid])
+        # the frame is compacted on the device (16 B per active particle)
+        engine.write_frame_compact(outdir, done, *sc.frame())
+        if num_write > 0:
+            d = sc.download()
+            for pid in range(min(num_write, spec.n)):
+                engine.write_pid_row(outdir, done, st["dt"], pid, d["x"][pid], d["v"][pid], rec["accel"][pid],
+                                     d["angv"][pid], rec["ang_accel"][pid], d["active"][pid])
         frames += 1
     if total > done:
         sc.step(total - done)

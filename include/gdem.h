@@ -262,6 +262,14 @@ int gdem_scene_download(gdem_scene* scene, gdem_particles_out* out);
  * particles (GDEM_E_ARG beyond).  Calls that overlap on one GPU share the
  * array and race. */
 int gdem_scene_download_noalloc(gdem_scene* scene, gdem_particles_out* out);
+/* The data of one solver_output frame (write_frame_bin,
+ * solver_output.cu:259-316), compacted on the device: the active particles'
+ * positions and radii in original id order, each value multiplied by
+ * length_factor in FP64 and converted to float32 there, so 16 B per active
+ * particle cross to the host instead of the whole state.  xyz_out holds 3 n
+ * floats and r_out n floats (n: the scene's particle count); the first
+ * *n_active entries are written.  gdem_write_frame_compact writes the file. */
+int gdem_scene_frame(gdem_scene* scene, double length_factor, float* xyz_out, float* r_out, int* n_active);
 /* Replaces deallocate(ParticleSys<DeviceMemory>*) and the free_* calls.
  * Destroy every scene before the context it was created on. */
 void gdem_scene_destroy(gdem_scene* scene);
@@ -365,9 +373,13 @@ int gdem_settings_wall(const char* path, int i, char* buf, int len);
 int gdem_settings_particles(const char* path, double* x, double* v);
 /* solver_output_init (solver_output.cu:14-41) */
 int gdem_output_init(const char* dir);
-/* write_frame_bin (solver_output.cu:259-316): dir/result_%012d.bin */
+/* write_frame_bin (solver_output.cu:259-316) from host state:
+ * dir/result_%012d.bin */
 int gdem_write_frame_bin(const char* dir, int step, int n, const double* x, const double* r,
                          const int* is_active, double length_factor);
+/* write_frame_bin (solver_output.cu:259-316) from the arrays gdem_scene_frame
+ * compacted on the device: the same file */
+int gdem_write_frame_compact(const char* dir, int step, int n_active, const float* xyz, const float* r);
 /* write_header_text (solver_output.cu:62-85): dir/pid<pid>.txt */
 int gdem_write_pid_header(const char* dir, int pid);
 /* write_single_text (solver_output.cu:112-172), unit factors */
