@@ -45,8 +45,8 @@ struct gdem_scene {
   uint32_t* h_abort = nullptr;        // pinned copy of the barrier's abort word
   hipEvent_t abort_ev = nullptr;      // recorded after that copy
   bool abort_probe = false;           // a copy is in flight or done
-  int* h_lstat = nullptr;             // pinned copy of the control words from CTL_NACT (active count,
-                                      // and at +3 the sum of the pair-list lengths of the last rebuild)
+  int* h_lstat = nullptr;             // pinned copy of the control words CTL_NACT .. CTL_NBSUM + 1 (the
+                                      // active count, and the 64-bit sum of the last rebuild's list lengths)
   bool lstat_probe = false;           // a copy is in flight or done (abort_ev follows it)
 
   ~gdem_scene() {
@@ -62,6 +62,9 @@ struct gdem_scene {
 
 namespace gdem {
 namespace {
+
+// control words CTL_NACT .. CTL_NBSUM + 1, copied to the host after each step call
+constexpr int LSTAT_WORDS = CTL_NBSUM + 2 - CTL_NACT;
 
 template <class H>
 int alloc_bufs(gdem_scene* sc, Bufs<H>& B, size_t ncell) {
@@ -174,7 +177,8 @@ int probe_barrier(gdem_scene* sc) {
   gdem_ctx* c = sc->ctx;
   if (sc->coop)
     HIPCHK(c, hipMemcpyAsync(sc->h_abort, bufs<H>(sc).gbar + GBAR_ABORT, 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(sc->h_lstat, sc->d_ctl + CTL_NACT, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sc->h_lstat, sc->d_ctl + CTL_NACT, LSTAT_WORDS * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipEventRecord(sc->abort_ev, c->stream));
   sc->abort_probe = sc->coop;
   sc->lstat_probe = true;
@@ -189,8 +193,16 @@ int probe_barrier(gdem_scene* sc) {
 // change drops the captured graph, which holds the other kernel.
 constexpr int LIST_PREFETCH_MIN = 4;
 
-void set_prefetch(gdem_scene* sc, long long nbsum, long long nact) {
-  const int want = nbsum >= (long long)LIST_PREFETCH_MIN * nact && nact > 0 ? 1 : 0;
+// the list-length sum held in control words CTL_NBSUM, CTL_NBSUM + 1, from a
+// copy of the control words that starts at word `first`
+unsigned long long nbsum_of(const int* words, int first) {
+  unsigned long long v = 0;
+  std::memcpy(&v, words + (CTL_NBSUM - first), sizeof(v));
+  return v;
+}
+
+void set_prefetch(gdem_scene* sc, unsigned long long nbsum, long long nact) {
+  const int want = nact > 0 && nbsum >= (unsigned long long)LIST_PREFETCH_MIN * (unsigned long long)nact ? 1 : 0;
   if (want == sc->P.list_prefetch) return;
   sc->P.list_prefetch = want;
   if (sc->graph) {
@@ -201,7 +213,7 @@ void set_prefetch(gdem_scene* sc, long long nbsum, long long nact) {
 
 void choose_prefetch(gdem_scene* sc) {
   if (!sc->lstat_probe || hipEventQuery(sc->abort_ev) != hipSuccess) return;
-  set_prefetch(sc, sc->h_lstat[3], sc->h_lstat[0]);
+  set_prefetch(sc, nbsum_of(sc->h_lstat, CTL_NACT), sc->h_lstat[0]);
 }
 
 template <class H>
@@ -338,7 +350,7 @@ int create_impl(gdem_scene* sc, const gdem_particles_desc* pd) {
   HIPCHK(c, hipMemcpy(ctl2, sc->d_ctl, sizeof(ctl2), hipMemcpyDeviceToHost));
   sc->rebuild_base = ctl2[CTL_REBUILDS];
   sc->P.list_prefetch = 1;
-  set_prefetch(sc, ctl2[CTL_NBSUM], ctl2[CTL_NACT]);
+  set_prefetch(sc, nbsum_of(ctl2, 0), ctl2[CTL_NACT]);
   return GDEM_OK;
 }
 
@@ -785,8 +797,8 @@ int gdem_scene_create(gdem_ctx* c, const gdem_particles_desc* pd, const gdem_mes
   HIPCHK(c, hipEventCreateWithFlags(&sc->abort_ev, hipEventDisableTiming));
   HIPCHK(c, hipHostMalloc((void**)&sc->h_abort, sizeof(uint32_t), hipHostMallocDefault));
   *sc->h_abort = 0;
-  HIPCHK(c, hipHostMalloc((void**)&sc->h_lstat, 4 * sizeof(int), hipHostMallocDefault));
-  for (int k = 0; k < 4; ++k) sc->h_lstat[k] = 0;
+  HIPCHK(c, hipHostMalloc((void**)&sc->h_lstat, LSTAT_WORDS * sizeof(int), hipHostMallocDefault));
+  for (int k = 0; k < LSTAT_WORDS; ++k) sc->h_lstat[k] = 0;
   int rc = sc->precision == GDEM_FP64 ? create_impl<double>(sc.get(), pd) : create_impl<float>(sc.get(), pd);
   if (rc) return rc;
   *out = sc.release();

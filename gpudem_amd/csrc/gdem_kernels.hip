@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_keys(SimParams P, Bufs<H> B) {
   if (!rebuild_due(B.ctl)) return;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < P.N) keys_one(P, B, s, B.ctl[CTL_CUR]);
-  if (s == 0) B.ctl[CTL_NBSUM] = 0;
+  if (s == 0) *reinterpret_cast<unsigned long long*>(&B.ctl[CTL_NBSUM]) = 0ull;
 }
 
 template <bool GATED>
@@ -438,10 +438,11 @@ __global__ __launch_bounds__(256) void k_cells(SimParams P, Bufs<H> B) {
 // Adds a thread's list lengths to the workgroup's sum in LDS (zeroed before
 // a barrier that precedes this call) and the workgroup's sum to the scene's
 // control word; every thread of the workgroup calls it.
-__device__ __forceinline__ void add_list_lengths(int* ctl_sum, int* wg_sum, int mine) {
+__device__ __forceinline__ void add_list_lengths(int* ctl, int* wg_sum, int mine) {
   if (mine) atomicAdd(wg_sum, mine);
   __syncthreads();
-  if (threadIdx.x == 0 && *wg_sum) atomicAdd(ctl_sum, *wg_sum);
+  if (threadIdx.x == 0 && *wg_sum)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&ctl[CTL_NBSUM]), (unsigned long long)*wg_sum);
 }
 
 // Pair lists of the sorted particles, at full occupancy (both the chain and
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(PL_BLOCK) void k_lists(SimParams P, Bufs<H> B) {
   int mine = 0;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.N; s += gridDim.x * blockDim.x)
     mine += pairlist_one<PL_BLOCK, LONG>(L, s, s < nact, lds);
-  add_list_lengths(&B.ctl[CTL_NBSUM], &nbsum, mine);
+  add_list_lengths(B.ctl, &nbsum, mine);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   __shared__ int nbsum;
   if (threadIdx.x == 0) {
     nbsum = 0;
-    if (b == 0) B.ctl[CTL_NBSUM] = 0;
+    if (b == 0) *reinterpret_cast<unsigned long long*>(&B.ctl[CTL_NBSUM]) = 0ull;
   }
   // keys of this workgroup's chunk, counted for the first pass as they are made
   rs_clear(dig);
@@ -620,7 +621,7 @@ __global__ __launch_bounds__(256) void k_rebuild_grid(SimParams P, Bufs<H> B) {
   const int nact = B.ctl[CTL_NACT];
   int mine = 0;
   for (int s = first; s < N; s += stride) mine += pairlist_one<256, LONG>(L, s, s < nact, lds.pl);
-  add_list_lengths(&B.ctl[CTL_NBSUM], &nbsum, mine);
+  add_list_lengths(B.ctl, &nbsum, mine);
   if (last_block(&B.ctl[CTL_DONE_WALL])) {
     B.ctl[CTL_DONE_WALL] = 0;
     B.ctl[CTL_REBUILDS] += 1;

@@ -63,7 +63,8 @@ enum Ctl : int {
   CTL_NACT = 6,      // active particles after the last re-sort
   CTL_DONE_WALL = 7,
   CTL_REBUILDS = 8,
-  CTL_NBSUM = 9,     // sum of the pair-list lengths built by the last rebuild
+  CTL_NBSUM = 10,    // words 10-11: the sum of the pair-list lengths built by the last rebuild
+                     // (unsigned 64-bit; the control block is 8-byte aligned)
   CTL_WORDS = 16
 };
 
