@@ -264,7 +264,7 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
 // Fused contact step of every running scene.
 template <class T, class H, class PRow, class WRow>
 __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
-  __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
+  __shared__ StepLds<T, RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int sid = __builtin_amdgcn_readfirstlane(B.wave_scene[__builtin_amdgcn_readfirstlane(s >> 6)]);
@@ -316,6 +316,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
   }
   const Own<H> own = load_own<false>(A, s, running);
   tile_p[threadIdx.x] = own.p;
+  stage_vw<T>(lds, own.v, own.w);
   __syncthreads();
   if (sid >= 0) {
     const BScene& S = B.scenes[sid];
@@ -407,7 +408,7 @@ void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st) {
   kb_permute<H><<<grid, 256, 0, st>>>(B);
   if (B.nb_cap <= PL_LDS_CAP) kb_lists<H, false><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
   else kb_lists<H, true><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
-  if (B.hist_cap <= BPCAP_LDS && B.whist_cap <= BWCAP_LDS)
+  if (hist_lds_fits(B.cap, B.hist_cap, B.whist_cap, BPCAP_LDS, BWCAP_LDS))
     kb_step<T, H, HistLds<H, BPCAP_LDS>, HistLds<H, BWCAP_LDS>><<<grid, 256, 0, st>>>(B);
   else
     kb_step<T, H, HistScan<H>, HistScan<H>><<<grid, 256, 0, st>>>(B);

@@ -329,11 +329,10 @@ struct HistLds {
   int n;           // live slots
   uint64_t marks;  // slots in contact this step
   int* k0;         // thread 0's column; thread t's column is k0 + t, slot q at [q * 256]
-  // ok: this lane's particle is active.  The wave loads as many slots as its
-  // longest row (never more than the capacity); lanes without a particle read
-  // slot row 0 of particle 0 instead, so every address is inside the [cap][N]
-  // array.  Slots past a lane's own row hold other keys and are never
-  // searched.
+  // ok: this lane's particle is active.  The wave loads as many slot rows as
+  // its longest row (never more than the capacity); in each, only the lanes
+  // whose own row reaches it fetch their key (issue).  Slots past a lane's
+  // own row hold zeros and are never searched.
   // prepare returns the number of slot rows the wave loads; issue loads them
   // (split so that the wave shuffles of several rows come before the first
   // LDS load: the compiler makes any LDS access after such a load wait for it)
@@ -344,12 +343,22 @@ struct HistLds {
     for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
     return min(__builtin_amdgcn_readfirstlane(wmax), min(cap, CAP));
   }
+  // Buffer loads: the [cap][N] key array is the buffer, each lane's slot is
+  // its one 32-bit vector offset and each row's offset is scalar, so the rows
+  // cost no vector address registers (the host selects this row type only
+  // when cap * N * 4 bytes fit the 32-bit buffer range).  A lane past its own
+  // row length gives an offset beyond the buffer: the load then fetches
+  // nothing and writes 0 to the (never searched) LDS word.
   __device__ __forceinline__ void issue(int wmax, bool ok) {
-    const int* src = gkey + (ok ? s : 0);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)gkey, (short)0, (int)((uint32_t)cap * (uint32_t)N * 4u), 0x00020000);
+    const uint32_t lane_off = (uint32_t)s * 4u;
+    const int mine = ok ? n : 0;
     int* dst = k0 + (threadIdx.x & ~63);  // this wave's part of slot row 0
     for (int q = 0; q < wmax; ++q)
-      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + (size_t)q * N),
-                                       (__attribute__((address_space(3))) void*)(dst + q * 256), 4, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(dst + q * 256), 4,
+                                               q < mine ? lane_off : 0xfffffff0u, (uint32_t)q * (uint32_t)N * 4u, 0,
+                                               0);
   }
   __device__ __forceinline__ void load(int nh, bool ok) { issue(prepare(nh, ok), ok); }
   // slot of `key` among the first n_o slots of the particle held by thread
@@ -361,6 +370,13 @@ struct HistLds {
     return -1;
   }
 };
+
+// Whether HistLds can serve a scene: the capacities fit its LDS columns and
+// every key row offset fits the 32-bit range of its buffer loads.
+__host__ __device__ inline bool hist_lds_fits(int N, int hist_cap, int whist_cap, int pcap_lds, int wcap_lds) {
+  const unsigned long long cap = (unsigned long long)(hist_cap > whist_cap ? hist_cap : whist_cap);
+  return hist_cap <= pcap_lds && whist_cap <= wcap_lds && cap * (unsigned long long)N * 4ull < 0xfffffff0ull;
+}
 
 template <class H>
 struct HistScan {
@@ -489,26 +505,61 @@ constexpr int step_waves() {
   return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
 }
 
+// The mixed form (T = float) also stages the workgroup's velocities and spins
+// in LDS as the float4 values its force arithmetic converts them to, so a
+// touching partner inside the tile is read from LDS rather than gathered
+// from memory (where its line has usually left the L2 since the workgroup
+// read it as an own record).  The FP64 form gathers them from memory: its
+// double records would not leave LDS room for three workgroups per CU.
+template <class T>
+constexpr bool vw_tile() {
+  return sizeof(T) == 4;
+}
+
 // Touching pairs of a particle whose list entries (partner slots) the
 // screening pass keeps in LDS for the force pass (later ones are read again
-// from the list).
-constexpr int HIT_STASH = 16;
+// from the list): 16 in FP64, 8 in the mixed form, whose LDS also holds the
+// velocity and spin tile; either way a workgroup takes 48 KB of LDS, so three
+// fit a CU (at 52 KB only two did: the mixed kernel took 134 against 121 us).
+template <class T>
+constexpr int hit_stash() {
+  return vw_tile<T>() ? 8 : 16;
+}
 
 // The hit stash is used only in the pair pass and the vertex/edge features
-// only in the wall pass, so they share their LDS: a thread's VE_MAX feature
-// words are its own HIT_STASH stash words (ve_slot), so no thread's features
+// only in the wall pass, so they share their LDS: a thread's ve_max<T>()
+// feature words are its own stash words (ve_slot), so no thread's features
 // overlap another thread's stash, whatever pass the other waves of the
 // workgroup are in.
-constexpr int VE_MAX = HIT_STASH;
+template <class T>
+constexpr int ve_max() {
+  return hit_stash<T>();
+}
 
-template <int PC, int WC>
+template <class T, int PC, int WC>
 struct StepLds {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
-  int hit[HIT_STASH][256];
+  int hit[hit_stash<T>()][256];
+  float4 tv[vw_tile<T>() ? 256 : 1];  // vx, vy, vz, 1/m of the tile's slots (mixed form)
+  float4 tw[vw_tile<T>() ? 256 : 1];  // wx, wy, wz of the tile's slots (mixed form)
 };
 
-// feature word u (< VE_MAX) of thread tx
+// Stages a thread's own velocity and spin in the tile (mixed form).
+template <class T, class Lds>
+__device__ __forceinline__ void stage_vw(Lds& lds, const double4& v, const double4& w) {
+  if constexpr (vw_tile<T>()) {
+    float4 a = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    float4 b = make_float4((float)w.x, (float)w.y, (float)w.z, 0.f);
+    // (opaque, so the force pass converts its own copies where it needs
+    // them instead of keeping these live from here)
+    asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(b.z));
+    lds.tv[threadIdx.x] = a;
+    lds.tw[threadIdx.x] = b;
+  }
+}
+
+// feature word u (< ve_max<T>()) of thread tx
 template <class Lds>
 __device__ __forceinline__ int* ve_slot(Lds& lds, int tx, int u) {
   return &lds.hit[u][tx];
@@ -781,6 +832,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
                                               const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
                                               uint32_t& err, bool& refresh) {
   using H4 = typename Vec4<H>::type;
+  constexpr int HS = hit_stash<T>();
+  constexpr int VEM = ve_max<T>();
+  constexpr bool VW = vw_tile<T>();
   const int tx = threadIdx.x;
   const int lane = __lane_id();
   const int wtx = tx - lane;  // thread index of lane 0 of this wave
@@ -879,7 +933,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   {
     PRow& row = prow;
     // screening: which candidates overlap (FP64 test of the reference)
-    // The first HIT_STASH touching entries go to this thread's LDS column.
+    // The first HS touching entries go to this thread's LDS column.
     uint64_t hits = 0;
     int nhit = 0;
     int* stash = &lds.hit[0][tx];
@@ -893,7 +947,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         const double R = ri + pj.w;
         if (distsq < R * R) {
           hits |= (uint64_t)1 << (q0 + u);
-          if (nhit < HIT_STASH) stash[nhit * 256] = ev[u];
+          if (nhit < HS) stash[nhit * 256] = ev[u];
           ++nhit;
         }
       }
@@ -946,12 +1000,16 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       int slot;     // history slot, -1 for a new contact
       int so;       // owner's slot
       int o, o_first, o_n;  // owner's lane, its first pair's index, its history length
+      bool in_t;    // (mixed form) the partner is in the workgroup's tile
       double4 pj, vj, wj;
       Del3<H> d;
     };
     // The partner's records are requested as soon as the list entry is
     // known, before the history key search (dependent LDS reads), which only
-    // the displacement read needs.
+    // the displacement read needs.  In the mixed form a partner inside the
+    // tile is read from LDS in pair_finish; its lane's memory reads go to the
+    // tile's first slot instead, one line the whole wave shares, so that the
+    // reads stay unconditional.
     auto pair_fetch = [&](const int base, PairIn& q) {
       const int g = base + lane;
       const bool has = g < total;
@@ -966,8 +1024,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o_n = __shfl(nh0, o);
       const int r = g - o_first;  // rank among the owner's touching pairs
       // (the LDS read is marked non-temporal so it stays an LDS instruction)
-      int e = ld_nt(&lds.hit[r < HIT_STASH ? r : HIT_STASH - 1][wtx + o]);
-      if (has && r >= HIT_STASH) {
+      int e = ld_nt(&lds.hit[r < HS ? r : HS - 1][wtx + o]);
+      if (has && r >= HS) {
         e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
         // wait for it here, inside the rare branch (see above)
         asm volatile("" ::"v"(e));
@@ -977,9 +1035,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       q.o_first = o_first;
       q.o_n = o_n;
       q.e = has ? e : so;
-      q.pj = pin[q.e];
-      q.vj = vin[q.e];
-      q.wj = win[q.e];
+      q.in_t = VW && (unsigned)(q.e - tile0) < 256u;
+      const int me = q.in_t ? tile0 : q.e;
+      q.pj = pin[me];
+      q.vj = vin[me];
+      q.wj = win[me];
       // (keeps the compiler from sinking the loads above below the search)
       asm volatile("" ::: "memory");
       q.slot = has ? row.find_in(wtx + o, so, o_n, e) : -1;
@@ -1002,7 +1062,25 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
       const int e = q.e;
       const int slot = q.slot;
-      const double4 pj = q.pj, vj = q.vj, wj = q.wj;
+      double4 pj = q.pj;
+      V3<T> vj{(T)q.vj.x, (T)q.vj.y, (T)q.vj.z}, wj{(T)q.wj.x, (T)q.wj.y, (T)q.wj.z};
+      T invmj = (T)q.vj.w;
+      if constexpr (VW) {
+        // The tile holds the same float values the conversions above make.
+        // Its reads follow them (the asm waits for the gathers and orders
+        // the LDS reads after it), so the converted values, not the double
+        // records, are what stays live beside them.
+        asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z), "v"(invmj), "v"(wj.x), "v"(wj.y), "v"(wj.z) : "memory");
+        const int te = (e - tile0) & 255;
+        const double4 pl = ld_nt(tile_p + te);
+        const float4 vl = ld_nt(lds.tv + te), wl = ld_nt(lds.tw + te);
+        if (q.in_t) {
+          pj = pl;
+          vj = V3<T>{vl.x, vl.y, vl.z};
+          invmj = vl.w;
+          wj = V3<T>{wl.x, wl.y, wl.z};
+        }
+      }
       // new contacts take their particle's next free slots in list order
       const uint64_t fresh = __ballot(has && slot < 0);
       const uint64_t before_me = lane_range(max(o_first - base, 0), lane);
@@ -1036,17 +1114,17 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       }
       const T delMag = (T)delMagD;
       const V3<T> del = scal(-delMag, n);
-      const V3<T> vr{ovi.x - (T)vj.x, ovi.y - (T)vj.y, ovi.z - (T)vj.z};
+      const V3<T> vr{ovi.x - vj.x, ovi.y - vj.y, ovi.z - vj.z};
       const T vrn = dot(vr, n);
       const V3<T> vn = scal(vrn, n);
-      const T meff = T(1) / ((T)oinvm + (T)vj.w);
+      const T meff = T(1) / ((T)oinvm + invmj);
       const T eta = oetac * sqrt(meff);
       V3<T> fn;
       fn.x = -ok * del.x - eta * vn.x;
       fn.y = -ok * del.y - eta * vn.y;
       fn.z = -ok * del.z - eta * vn.z;
       const T rj = (T)pj.w;
-      V3<T> vrot{oriT * owi.x + rj * (T)wj.x, oriT * owi.y + rj * (T)wj.y, oriT * owi.z + rj * (T)wj.z};
+      V3<T> vrot{oriT * owi.x + rj * wj.x, oriT * owi.y + rj * wj.y, oriT * owi.z + rj * wj.z};
       vrot = cross(vrot, n);
       const V3<T> vt = vadd(vsub(vr, vn), vrot);
       const V3<T> old = slot >= 0 ? V3<T>{(T)q.d.x, (T)q.d.y, (T)q.d.z} : V3<T>{T(0), T(0), T(0)};
@@ -1120,7 +1198,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // (every lane's pair pass is done before the wall pass writes the union)
     __builtin_amdgcn_wave_barrier();
     // features of this particle's kept vertex/edge contacts of earlier rounds:
-    // *ve_slot(lds, tx, u), u < min(nve, VE_MAX)
+    // *ve_slot(lds, tx, u), u < min(nve, VEM)
     int nve = 0;
     int added = 0;  // new history slots this particle's contacts have taken
     uint64_t marks = 0;
@@ -1167,7 +1245,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int c_lo = max(o_first - base, 0);
       bool dup = false;
       if (con && hit != -1) {
-        for (int z = 0; z < min(o_nve, VE_MAX); ++z)
+        for (int z = 0; z < min(o_nve, VEM); ++z)
           if (*ve_slot(lds, wtx + o, z) == hit) dup = true;
       }
       for (int kk = 1; kk < span; ++kk) {
@@ -1238,7 +1316,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           mom.z += c2;
           if (ws >= 0) marks |= (uint64_t)1 << ws;
           if (hh != -1) {
-            if (nve < VE_MAX) *ve_slot(lds, tx, nve) = hh;
+            if (nve < VEM) *ve_slot(lds, tx, nve) = hh;
             else err |= GDEM_ERR_VE_OVERFLOW_BIT;
             ++nve;
           }

@@ -19,7 +19,7 @@ namespace gdem {
 // ---------------------------------------------------------------------------
 template <class T, class H, bool REC, bool PRE, class PRow, class WRow>
 __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
-  __shared__ StepLds<RowCap<PRow>::value, RowCap<WRow>::value> lds;
+  __shared__ StepLds<T, RowCap<PRow>::value, RowCap<WRow>::value> lds;
   const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int cur = __builtin_amdgcn_readfirstlane(B.ctl[CTL_CUR]);
   const int hc = __builtin_amdgcn_readfirstlane(B.ctl[CTL_HCUR]);
@@ -62,6 +62,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   const bool valid = s < P.N;
   const Own<H> own = load_own<PRE>(A, s, valid);
   tile_p[threadIdx.x] = own.p;
+  stage_vw<T>(lds, own.v, own.w);
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
@@ -815,7 +816,6 @@ __global__ __launch_bounds__(256) void k_set_history(SimParams P, Bufs<H> B, His
   const int N = P.N;
   const int hc = B.ctl[CTL_HCUR];
   const int id = B.pid[hc][s];
-  using H4 = typename Vec4<H>::type;
   int nh = in.nhist[id];
   if (nh > P.hist_cap) {
     atomicOr((unsigned int*)&B.ctl[CTL_ERR], GDEM_ERR_HIST_OVERFLOW_BIT);
@@ -919,7 +919,7 @@ template <class T, class H>
 void launch_step(const SimParams& P, const Bufs<H>& B, hipStream_t st) {
   Rec<T> none{};
   const int g = blocks_for(P.N, 256);
-  const bool lds_rows = P.hist_cap <= PCAP_REG && P.whist_cap <= WCAP_REG;
+  const bool lds_rows = hist_lds_fits(P.N, P.hist_cap, P.whist_cap, PCAP_REG, WCAP_REG);
   if (lds_rows && P.list_prefetch)
     k_step<T, H, false, true, HistLds<H, PCAP_REG>, HistLds<H, WCAP_REG>><<<g, 256, 0, st>>>(P, B, none);
   else if (lds_rows)
