@@ -758,6 +758,33 @@ __device__ __forceinline__ int wave_incl_sum(int v) {
   return v;
 }
 
+// The same for 0 <= v < 128, bit-sliced over ballots and lane-mask counts:
+// VALU and scalar work only, where the shuffle form waits on six dependent
+// cross-lane (LDS-unit) permutes.
+__device__ __forceinline__ int wave_incl_sum7(int v) {
+  int sum = 0;
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {
+    const int bit = (v >> b) & 1;
+    const uint64_t m = __ballot(bit);
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    sum += (below + bit) << b;
+  }
+  return sum;
+}
+
+// Maximum over the wave of 0 <= v < 128 (wave-uniform), bit by bit from the
+// top with ballots.
+__device__ __forceinline__ int wave_max7(int v) {
+  int m = 0;
+#pragma unroll
+  for (int b = 6; b >= 0; --b) {
+    const int c = m | (1 << b);
+    if (__ballot(v >= c) != 0) m = c;
+  }
+  return m;
+}
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
@@ -887,7 +914,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
 
   // The wave's wall candidates are spread over its lanes, 64 at a time; the
   // first 64 are fetched now so the load overlaps the pair pass.
-  const int w_incl = wave_incl_sum(nwl);
+  const int w_incl = wave_incl_sum7(nwl);
   const int w_total = __builtin_amdgcn_readfirstlane(__shfl(w_incl, 63));
   int w_t0 = 0;
   {
@@ -982,7 +1009,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     rows_ready(wrow);
     // the touching pairs, spread over the lanes
     const int cnt = __popcll(hits);
-    const int incl = wave_incl_sum(cnt);
+    const int incl = wave_incl_sum7(cnt);
     const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
     const int first = incl - cnt;
     int added = 0;  // new history slots this particle's pairs have asked for
@@ -1135,7 +1162,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       // every particle adds its pairs of this round in list order
       const int my_lo = max(first - base, 0), my_hi = min(first + cnt - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
-      const int rounds = wave_max(my_n);
+      const int rounds = wave_max7(my_n);
       for (int t = 0; t < rounds; ++t) {
         const bool mine = t < my_n;
         const int src = mine ? my_lo + t : lane;
@@ -1241,7 +1268,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       // the candidate's owner's are [c_lo, ...)
       const int my_lo = max(first - base, 0), my_hi = min(first + nwl - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
-      const int span = wave_max(my_n);
+      const int span = wave_max7(my_n);
       const int c_lo = max(o_first - base, 0);
       bool dup = false;
       if (con && hit != -1) {
