@@ -363,10 +363,20 @@ struct HistLds {
   __device__ __forceinline__ void load(int nh, bool ok) { issue(prepare(nh, ok), ok); }
   // slot of `key` among the first n_o slots of the particle held by thread
   // col (slot index so)
+  // (keys of a row are distinct; the slots are read four at a time, so a
+  // search costs one LDS round trip per four slots, not one per slot)
   __device__ __forceinline__ int find_in(int col, int so, int n_o, int key) const {
     const int* c = k0 + col;
-    for (int q = 0; q < n_o; ++q)
-      if (c[q * 256] == key) return q;
+    for (int q0 = 0; q0 < n_o; q0 += 4) {
+      int hit = -1;
+#pragma unroll
+      for (int u = 3; u >= 0; --u) {
+        const int q = q0 + u;
+        const int kq = c[(q < CAP ? q : CAP - 1) * 256];
+        if (q < n_o && kq == key) hit = q;
+      }
+      if (hit >= 0) return hit;
+    }
     return -1;
   }
 };
@@ -792,13 +802,17 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // Lane owning work item g: the first lane whose inclusive count exceeds g
-// (incl is nondecreasing over the lanes; the result is 0..63 for any g).
+// (incl is nondecreasing over the lanes; the result is 0..63 for any g, 63
+// when no lane's count exceeds g).  A 4-way search: three levels of three
+// independent shuffles, instead of six dependent ones.
 __device__ __forceinline__ int owner_lane(int incl, int g) {
   int lo = 0;
 #pragma unroll
-  for (int step = 32; step > 0; step >>= 1) {
-    const int v = __shfl(incl, lo + step - 1);
-    if (v <= g) lo += step;
+  for (int span = 16; span > 0; span >>= 2) {
+    const int v1 = __shfl(incl, lo + span - 1);
+    const int v2 = __shfl(incl, lo + 2 * span - 1);
+    const int v3 = __shfl(incl, lo + 3 * span - 1);
+    lo += ((v1 <= g) + (v2 <= g) + (v3 <= g)) * span;
   }
   return lo;
 }
@@ -1163,11 +1177,19 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int my_lo = max(first - base, 0), my_hi = min(first + cnt - base, 64);
       const int my_n = max(my_hi - my_lo, 0);
       const int rounds = wave_max7(my_n);
+      // The mixed form adds each pair's fn + ft as one vector (six values
+      // fetched per pair instead of nine); FP64 adds fn, then ft, as the
+      // reference's accumulation does.
+      constexpr bool FOLD = sizeof(T) == 4 && !REC;
+      const V3<T> fa = FOLD ? vadd(fn, ft) : fn;
       for (int t = 0; t < rounds; ++t) {
         const bool mine = t < my_n;
         const int src = mine ? my_lo + t : lane;
-        const T a0 = __shfl(fn.x, src), a1 = __shfl(fn.y, src), a2 = __shfl(fn.z, src);
-        const T b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        const T a0 = __shfl(fa.x, src), a1 = __shfl(fa.y, src), a2 = __shfl(fa.z, src);
+        T b0 = T(0), b1 = T(0), b2 = T(0);
+        if constexpr (!FOLD) {
+          b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        }
         const T c0 = __shfl(mm.x, src), c1 = __shfl(mm.y, src), c2 = __shfl(mm.z, src);
         const int ws = __shfl(wslot, src);
         int rkey = 0;
@@ -1180,9 +1202,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           f.x += a0;
           f.y += a1;
           f.z += a2;
-          f.x += b0;
-          f.y += b1;
-          f.z += b2;
+          if constexpr (!FOLD) {
+            f.x += b0;
+            f.y += b1;
+            f.z += b2;
+          }
           mom.x += c0;
           mom.y += c1;
           mom.z += c2;
@@ -1317,12 +1341,18 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
         mm = scal(oriT, cross(n, ft));
       }
       // every particle adds its kept contacts of this round in list order
+      // (fn + ft folded in the mixed form, as for the pairs)
+      constexpr bool FOLD = sizeof(T) == 4 && !REC;
+      const V3<T> fa = FOLD ? vadd(fn, ft) : fn;
       for (int u = 0; u < span; ++u) {
         const bool mine = u < my_n;
         const int src = mine ? my_lo + u : lane;
         const int kp = __shfl((int)keep, src);
-        const T a0 = __shfl(fn.x, src), a1 = __shfl(fn.y, src), a2 = __shfl(fn.z, src);
-        const T b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        const T a0 = __shfl(fa.x, src), a1 = __shfl(fa.y, src), a2 = __shfl(fa.z, src);
+        T b0 = T(0), b1 = T(0), b2 = T(0);
+        if constexpr (!FOLD) {
+          b0 = __shfl(ft.x, src), b1 = __shfl(ft.y, src), b2 = __shfl(ft.z, src);
+        }
         const T c0 = __shfl(mm.x, src), c1 = __shfl(mm.y, src), c2 = __shfl(mm.z, src);
         const int ws = __shfl(wslot, src), hh = __shfl(hit, src);
         int rkey = 0;
@@ -1335,9 +1365,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
           f.x += a0;
           f.y += a1;
           f.z += a2;
-          f.x += b0;
-          f.y += b1;
-          f.z += b2;
+          if constexpr (!FOLD) {
+            f.x += b0;
+            f.y += b1;
+            f.z += b2;
+          }
           mom.x += c0;
           mom.y += c1;
           mom.z += c2;
