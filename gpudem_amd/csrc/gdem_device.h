@@ -546,13 +546,19 @@ constexpr int ve_max() {
   return hit_stash<T>();
 }
 
+template <bool VW>
+struct VwTile {
+  float4 tv[256];  // vx, vy, vz, 1/m of the tile's slots
+  float4 tw[256];  // wx, wy, wz of the tile's slots
+};
+template <>
+struct VwTile<false> {};
+
 template <class T, int PC, int WC>
-struct StepLds {
+struct StepLds : VwTile<vw_tile<T>()> {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
   int hit[hit_stash<T>()][256];
-  float4 tv[vw_tile<T>() ? 256 : 1];  // vx, vy, vz, 1/m of the tile's slots (mixed form)
-  float4 tw[vw_tile<T>() ? 256 : 1];  // wx, wy, wz of the tile's slots (mixed form)
 };
 
 // Stages a thread's own velocity and spin in the tile (mixed form).
@@ -1097,7 +1103,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       // the owner's position record, from the workgroup's tile
       const double4 op = ld_nt(tile_p + wtx + o);
       const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
-      const double oinvm = __shfl(invm, o);
+      const T oinvm = __shfl((T)invm, o);
       const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
       const V3<T> ovi{__shfl(vi.x, o), __shfl(vi.y, o), __shfl(vi.z, o)};
       const V3<T> owi{__shfl(wi.x, o), __shfl(wi.y, o), __shfl(wi.z, o)};
@@ -1158,7 +1164,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const V3<T> vr{ovi.x - vj.x, ovi.y - vj.y, ovi.z - vj.z};
       const T vrn = dot(vr, n);
       const V3<T> vn = scal(vrn, n);
-      const T meff = T(1) / ((T)oinvm + invmj);
+      const T meff = T(1) / (oinvm + invmj);
       const T eta = oetac * sqrt(meff);
       V3<T> fn;
       fn.x = -ok * del.x - eta * vn.x;
