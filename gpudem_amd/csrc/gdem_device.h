@@ -424,6 +424,21 @@ __device__ __forceinline__ void rows_ready(HistLds<H, CAP>&) {
 // (device_dem.cu:654-737): the displacement `old` advanced by vt dt and
 // projected onto the tangent plane, the Coulomb-capped spring force, and the
 // displacement to remember (returned in dn).
+// Reciprocal and square root of the contact force arithmetic: the
+// reference's correctly rounded operations in FP64, the hardware's one-ulp
+// instructions (v_rcp_f32, v_sqrt_f32) in the mixed form, whose forces carry
+// a relative error of order 1e-7 either way.
+template <class T>
+__device__ __forceinline__ T frcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+template <class T>
+__device__ __forceinline__ T fsqrt(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
+}
+
 template <class T>
 __device__ __forceinline__ V3<T> tangential_force(V3<T> old, V3<T> fn, V3<T> vt, V3<T> n, T eta, T k, T mu, T dt,
                                                   V3<T>& dn_out) {
@@ -435,10 +450,11 @@ __device__ __forceinline__ V3<T> tangential_force(V3<T> old, V3<T> fn, V3<T> vt,
   const T fnsq = dot(fn, fn);
   if (ftsq > (mu * mu) * fnsq) {
     if (ftsq != T(0)) {
-      const V3<T> t = scal(T(1) / (sqrt(ftsq) + T(1e-15)), ft);
-      const T fnn = sqrt(fnsq);
+      const V3<T> t = scal(frcp(fsqrt(ftsq) + T(1e-15)), ft);
+      const T fnn = fsqrt(fnsq);
       ft = scal(mu * fnn, t);
-      dn = scal(T(-1) / k, ft);
+      if constexpr (sizeof(T) == 4) dn = scal(-frcp(k), ft);
+      else dn = scal(T(-1) / k, ft);
     } else {
       ft = V3<T>{T(0), T(0), T(0)};
     }
@@ -561,17 +577,18 @@ struct StepLds : VwTile<vw_tile<T>()> {
   int hit[hit_stash<T>()][256];
 };
 
-// Stages a thread's own velocity and spin in the tile (mixed form).
+// Stages a thread's own velocity and spin in the tile (mixed form); t is its
+// slot's place in the tile.
 template <class T, class Lds>
-__device__ __forceinline__ void stage_vw(Lds& lds, const double4& v, const double4& w) {
+__device__ __forceinline__ void stage_vw(Lds& lds, int t, const double4& v, const double4& w) {
   if constexpr (vw_tile<T>()) {
     float4 a = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
     float4 b = make_float4((float)w.x, (float)w.y, (float)w.z, 0.f);
     // (opaque, so the force pass converts its own copies where it needs
     // them instead of keeping these live from here)
     asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(b.z));
-    lds.tv[threadIdx.x] = a;
-    lds.tw[threadIdx.x] = b;
+    lds.tv[t] = a;
+    lds.tw[t] = b;
   }
 }
 
@@ -870,10 +887,11 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // edge deduplication needs that order).
 //
 // own holds the particle's records (load_own, issued by the caller before it
-// stages own.p in tile_p[threadIdx.x]); tile_p holds the position records of
+// stages own.p in tile_p[s - tile0]); tile_p holds the position records of
 // slots [tile0, tile0 + 256), this workgroup's particles, so neighbours inside
 // the tile (most of them, in Morton order) are read from LDS instead of
-// memory.
+// memory.  Which thread holds which slot of the tile is the caller's choice;
+// lane 0 must hold the lowest slot of its wave.
 template <class T, class H, bool REC, bool PRE, class PRow, class WRow, class Lds>
 __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, const Own<H>& own,
                                               const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
@@ -885,6 +903,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const int tx = threadIdx.x;
   const int lane = __lane_id();
   const int wtx = tx - lane;  // thread index of lane 0 of this wave
+  // lane 0's slot: the lowest of the wave, so a particle's whenever any lane
+  // of the wave holds one
+  const int s_lane0 = __shfl(s, 0);
   const int N = A.N;
   const double4* __restrict__ pin = A.pin;
   const double4* __restrict__ vin = A.vin;
@@ -1067,7 +1088,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       // whenever the wave has pairs.  (The shuffle runs on every lane: in a
       // branch it would read lanes that are off in it, and get nothing.)
       const int so_o = __shfl(s, o);
-      const int so = has ? so_o : s - lane;
+      const int so = has ? so_o : s_lane0;
       const int o_n = __shfl(nh0, o);
       const int r = g - o_first;  // rank among the owner's touching pairs
       // (the LDS read is marked non-temporal so it stays an LDS instruction)
@@ -1101,7 +1122,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o_n = q.o_n;
       const int o_added = __shfl(added, o);
       // the owner's position record, from the workgroup's tile
-      const double4 op = ld_nt(tile_p + wtx + o);
+      const double4 op = ld_nt(tile_p + (so - tile0));
       const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
       const T oinvm = __shfl((T)invm, o);
       const T ok = __shfl(k, o), oetac = __shfl(etac, o), oriT = __shfl(riT, o);
@@ -1156,7 +1177,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       if constexpr (sizeof(T) == 8) {
         n = V3<T>{dx / dist, dy / dist, dz / dist};
       } else {
-        const T inv = T(1) / (T)dist;
+        const T inv = frcp((T)dist);
         n = V3<T>{(T)dx * inv, (T)dy * inv, (T)dz * inv};
       }
       const T delMag = (T)delMagD;
@@ -1164,8 +1185,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const V3<T> vr{ovi.x - vj.x, ovi.y - vj.y, ovi.z - vj.z};
       const T vrn = dot(vr, n);
       const V3<T> vn = scal(vrn, n);
-      const T meff = T(1) / (oinvm + invmj);
-      const T eta = oetac * sqrt(meff);
+      const T meff = frcp(oinvm + invmj);
+      const T eta = oetac * fsqrt(meff);
       V3<T> fn;
       fn.x = -ok * del.x - eta * vn.x;
       fn.y = -ok * del.y - eta * vn.y;
@@ -1268,7 +1289,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
       const int so = __shfl(s, o);
-      const double4 op = ld_nt(tile_p + wtx + o);  // the owner's position record
+      const double4 op = ld_nt(tile_p + (so - tile0));  // the owner's position record
       const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
       const double orsq = ori * ori;
       const int o_n = __shfl(nw0, o), o_added = __shfl(added, o), o_nve = __shfl(nve, o);
@@ -1432,7 +1453,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   double4 p1, v1, w1;
   if constexpr (sizeof(T) == 8) {
     asm volatile("" ::: "memory");
-    p1 = ld_nt(tile_p + tx);
+    p1 = ld_nt(tile_p + (s - tile0));
     v1 = vin[s];
     w1 = win[s];
   } else {

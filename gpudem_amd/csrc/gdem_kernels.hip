@@ -20,7 +20,10 @@ namespace gdem {
 template <class T, class H, bool REC, bool PRE, class PRow, class WRow>
 __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs<H> B, Rec<T> rec) {
   __shared__ StepLds<T, RowCap<PRow>::value, RowCap<WRow>::value> lds;
-  const int s = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  // thread tt of the workgroup holds slot tt of the tile
+  const int tile0 = xcd_tile(blockIdx.x, gridDim.x) * 256;
+  const int tt = threadIdx.x;
+  const int s = tile0 + tt;
   const int cur = __builtin_amdgcn_readfirstlane(B.ctl[CTL_CUR]);
   const int hc = __builtin_amdgcn_readfirstlane(B.ctl[CTL_HCUR]);
   StepArgs<H> A;
@@ -61,12 +64,12 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   __shared__ double4 tile_p[256];
   const bool valid = s < P.N;
   const Own<H> own = load_own<PRE>(A, s, valid);
-  tile_p[threadIdx.x] = own.p;
-  stage_vw<T>(lds, own.v, own.w);
+  tile_p[tt] = own.p;
+  stage_vw<T>(lds, tt, own.v, own.w);
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
-  step_particle<T, H, REC, PRE, PRow, WRow>(A, s, valid, own, tile_p, s - (int)threadIdx.x, lds, rec, err, refresh);
+  step_particle<T, H, REC, PRE, PRow, WRow>(A, s, valid, own, tile_p, tile0, lds, rec, err, refresh);
   if (__ballot(refresh) != 0 && __lane_id() == 0) B.ctl[CTL_REFRESH] = 1;
   if (err) atomicOr((unsigned int*)&B.ctl[CTL_ERR], err);
   if (last_block(&B.ctl[CTL_DONE_STEP])) {
