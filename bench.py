@@ -301,6 +301,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2000,
                     help="timed steps per scene of the plain single-scene comparison")
     ap.add_argument("--plain-sample", type=int, default=32, help="scenes of the mix timed on the plain path")
+    ap.add_argument("--plain-graph-steps", type=int, default=50,
+                    help="steps per hipGraph of the plain single-scene comparison (0: direct launches)")
     ap.add_argument("--served-requests", type=int, default=60)
     ap.add_argument("--served-budget-gb", type=float, default=8.0)
     ap.add_argument("--no-batch", action="store_true")
@@ -441,8 +443,10 @@ def main():
             bf = run_batched(ctx, mix, GDEM_FP64, tuned, args.batch_steps)
             log(f"# batched fp64: {bf['value']:.3e} particle-steps/s")
         sample = mix[::max(1, len(mix) // args.plain_sample)][:args.plain_sample]
-        pl = run_plain(ctx, sample, GDEM_FP64, bm["lead_steps"], args.plain_steps, graph_steps=50)
-        log(f"# plain fp64 (single-scene path, hipGraph, {len(sample)} scenes one at a time): {pl['value']:.3e}")
+        pl = run_plain(ctx, sample, GDEM_FP64, bm["lead_steps"], args.plain_steps,
+                       graph_steps=args.plain_graph_steps)
+        log(f"# plain fp64 (single-scene path, {args.plain_graph_steps}-step hipGraphs, {len(sample)} scenes one at "
+            f"a time): {pl['value']:.3e}")
         ok = ok and bm["errors"] == 0 and (bf is None or bf["errors"] == 0)
         out["batched"] = {"workload": "box_mix_256_logU1e3-5e4", "value": bm["value"],
                           "unit": "particle-steps/s", "dtype": "f32 forces / f64 integration",
