@@ -890,8 +890,8 @@ __device__ __forceinline__ uint64_t lane_range(int lo, int hi) {
 // stages own.p in tile_p[s - tile0]); tile_p holds the position records of
 // slots [tile0, tile0 + 256), this workgroup's particles, so neighbours inside
 // the tile (most of them, in Morton order) are read from LDS instead of
-// memory.  Which thread holds which slot of the tile is the caller's choice;
-// lane 0 must hold the lowest slot of its wave.
+// memory.  The lanes of a wave hold consecutive slots (lane i: the wave's
+// lowest slot + i).
 template <class T, class H, bool REC, bool PRE, class PRow, class WRow, class Lds>
 __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool valid, const Own<H>& own,
                                               const double4* tile_p, int tile0, Lds& lds, const Rec<T>& rec,
@@ -904,7 +904,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   const int lane = __lane_id();
   const int wtx = tx - lane;  // thread index of lane 0 of this wave
   // lane 0's slot: the lowest of the wave, so a particle's whenever any lane
-  // of the wave holds one
+  // of the wave holds one; lane i holds slot s_lane0 + i
   const int s_lane0 = __shfl(s, 0);
   const int N = A.N;
   const double4* __restrict__ pin = A.pin;
@@ -961,7 +961,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   {
     // shuffles run with every lane active
     const int o = owner_lane(w_incl, lane);
-    const int o_first = __shfl(w_incl - nwl, o), so = __shfl(s, o);
+    const int o_first = __shfl(w_incl - nwl, o), so = s_lane0 + o;
     if (lane < w_total) w_t0 = ld_nt(A.wl + (size_t)(lane - o_first) * N + so);
   }
 
@@ -1083,20 +1083,23 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const bool has = g < total;
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
-      const uint64_t o_hits = shfl64(hits, o);
       // A lane without a pair uses the slot of lane 0, which holds a particle
-      // whenever the wave has pairs.  (The shuffle runs on every lane: in a
-      // branch it would read lanes that are off in it, and get nothing.)
-      const int so_o = __shfl(s, o);
-      const int so = has ? so_o : s_lane0;
+      // whenever the wave has pairs.
+      const int so = has ? s_lane0 + o : s_lane0;
       const int o_n = __shfl(nh0, o);
       const int r = g - o_first;  // rank among the owner's touching pairs
       // (the LDS read is marked non-temporal so it stays an LDS instruction)
       int e = ld_nt(&lds.hit[r < HS ? r : HS - 1][wtx + o]);
-      if (has && r >= HS) {
-        e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
-        // wait for it here, inside the rare branch (see above)
-        asm volatile("" ::"v"(e));
+      // Pairs past the stash: the entry is read from the list.  (The branch
+      // on the ballot is uniform, so the shuffle inside runs on every lane:
+      // in a divergent branch it would read lanes that are off in it.)
+      if (__ballot(has && r >= HS) != 0) {
+        const uint64_t o_hits = shfl64(hits, o);
+        if (has && r >= HS) {
+          e = A.nbr[(size_t)nth_bit(o_hits, r) * N + so];
+          // wait for it here, inside the rare branch (see above)
+          asm volatile("" ::"v"(e));
+        }
       }
       q.so = so;
       q.o = o;
@@ -1288,7 +1291,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       const bool has = g < total;
       const int o = owner_lane(incl, g);
       const int o_first = __shfl(first, o);
-      const int so = __shfl(s, o);
+      const int so = s_lane0 + o;
       const double4 op = ld_nt(tile_p + (so - tile0));  // the owner's position record
       const double opx = op.x, opy = op.y, opz = op.z, ori = op.w;
       const double orsq = ori * ori;
@@ -1430,7 +1433,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     for (int base = 64; base < total; base += 64) {
       const int g = base + lane;
       const int o = owner_lane(incl, g);
-      const int o_first = __shfl(first, o), so = __shfl(s, o);
+      const int o_first = __shfl(first, o), so = s_lane0 + o;
       const int tn = g < total ? ld_nt(A.wl + (size_t)(g - o_first) * N + so) : 0;
       wall_round(base, tn);
     }
