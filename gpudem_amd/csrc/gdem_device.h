@@ -531,12 +531,14 @@ constexpr int step_waves() {
   return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
 }
 
-// The mixed form (T = float) also stages the workgroup's velocities and spins
-// in LDS as the float4 values its force arithmetic converts them to, so a
-// touching partner inside the tile is read from LDS rather than gathered
-// from memory (where its line has usually left the L2 since the workgroup
-// read it as an own record).  The FP64 form gathers them from memory: its
-// double records would not leave LDS room for three workgroups per CU.
+// The workgroup's velocities (and, in the mixed form, spins) are staged in
+// LDS with the positions, so a touching partner inside the tile is read from
+// LDS rather than gathered from memory (where its line has usually left the
+// L2 since the workgroup read it as an own record).  The mixed form stages
+// the float4 values its force arithmetic converts them to; the FP64 form
+// stages the velocity records themselves and gathers spins from memory, as
+// its double records would not leave LDS room for both at three workgroups
+// per CU.
 template <class T>
 constexpr bool vw_tile() {
   return sizeof(T) == 4;
@@ -544,12 +546,12 @@ constexpr bool vw_tile() {
 
 // Touching pairs of a particle whose list entries (partner slots) the
 // screening pass keeps in LDS for the force pass (later ones are read again
-// from the list): 16 in FP64, 8 in the mixed form, whose LDS also holds the
-// velocity and spin tile; either way a workgroup takes 48 KB of LDS, so three
-// fit a CU (at 52 KB only two did: the mixed kernel took 134 against 121 us).
+// from the list): 8, so that with the velocity tile a workgroup takes 48 KB
+// of LDS and three fit a CU (at 52 KB only two did: the mixed kernel took 134
+// against 121 us).
 template <class T>
 constexpr int hit_stash() {
-  return vw_tile<T>() ? 8 : 16;
+  return 8;
 }
 
 // The hit stash is used only in the pair pass and the vertex/edge features
@@ -562,26 +564,30 @@ constexpr int ve_max() {
   return hit_stash<T>();
 }
 
-template <bool VW>
+template <class T>
 struct VwTile {
   float4 tv[256];  // vx, vy, vz, 1/m of the tile's slots
   float4 tw[256];  // wx, wy, wz of the tile's slots
 };
 template <>
-struct VwTile<false> {};
+struct VwTile<double> {
+  double4 tv[256];  // velocity records of the tile's slots
+};
 
 template <class T, int PC, int WC>
-struct StepLds : VwTile<vw_tile<T>()> {
+struct StepLds : VwTile<T> {
   int pkey[PC > 0 ? PC : 1][256];
   int wkey[WC > 0 ? WC : 1][256];
   int hit[hit_stash<T>()][256];
 };
 
-// Stages a thread's own velocity and spin in the tile (mixed form); t is its
-// slot's place in the tile.
+// Stages a thread's own velocity (and spin) in the tile; t is its slot's
+// place in the tile.
 template <class T, class Lds>
 __device__ __forceinline__ void stage_vw(Lds& lds, int t, const double4& v, const double4& w) {
-  if constexpr (vw_tile<T>()) {
+  if constexpr (!vw_tile<T>()) {
+    lds.tv[t] = v;
+  } else {
     float4 a = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
     float4 b = make_float4((float)w.x, (float)w.y, (float)w.z, 0.f);
     // (opaque, so the force pass converts its own copies where it needs
@@ -1068,7 +1074,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       int slot;     // history slot, -1 for a new contact
       int so;       // owner's slot
       int o, o_first, o_n;  // owner's lane, its first pair's index, its history length
-      bool in_t;    // (mixed form) the partner is in the workgroup's tile
+      bool in_t;    // the partner is in the workgroup's tile
       double4 pj, vj, wj;
       Del3<H> d;
     };
@@ -1106,11 +1112,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       q.o_first = o_first;
       q.o_n = o_n;
       q.e = has ? e : so;
-      q.in_t = VW && (unsigned)(q.e - tile0) < 256u;
+      q.in_t = (unsigned)(q.e - tile0) < 256u;
       const int me = q.in_t ? tile0 : q.e;
       q.pj = pin[me];
       q.vj = vin[me];
-      q.wj = win[me];
+      q.wj = win[VW ? me : q.e];
       // (keeps the compiler from sinking the loads above below the search)
       asm volatile("" ::: "memory");
       q.slot = has ? row.find_in(wtx + o, so, o_n, e) : -1;
@@ -1136,7 +1142,16 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       double4 pj = q.pj;
       V3<T> vj{(T)q.vj.x, (T)q.vj.y, (T)q.vj.z}, wj{(T)q.wj.x, (T)q.wj.y, (T)q.wj.z};
       T invmj = (T)q.vj.w;
-      if constexpr (VW) {
+      if constexpr (!VW) {
+        const int te = (e - tile0) & 255;
+        const double4 pl = ld_nt(tile_p + te);
+        const double4 vl = ld_nt(lds.tv + te);
+        if (q.in_t) {
+          pj = pl;
+          vj = V3<T>{vl.x, vl.y, vl.z};
+          invmj = vl.w;
+        }
+      } else {
         // The tile holds the same float values the conversions above make.
         // Its reads follow them (the asm waits for the gathers and orders
         // the LDS reads after it), so the converted values, not the double
