@@ -109,6 +109,7 @@ struct gdem_batch {
   gdem::SlotCtl* d_sctl = nullptr;
   int* d_ctl = nullptr;
   int* d_wave = nullptr;
+  int* d_due = nullptr;
   gdem::TriGeo* d_tri = nullptr;
   double* d_tbox = nullptr;
   int* d_wc_off = nullptr;
@@ -204,6 +205,7 @@ int alloc_batch(gdem_batch* b) {
   B.wc_off = b->d_wc_off;
   B.wc_tri = b->d_wc_tri;
   B.wave_scene = b->d_wave;
+  B.due = b->d_due;
   B.scenes = b->d_scenes;
   B.sctl = b->d_sctl;
   B.ctl = b->d_ctl;
@@ -481,6 +483,7 @@ int gdem_batch_create(gdem_ctx* c, const gdem_batch_params* bp, gdem_batch** out
   HIPCHK(c, A.alloc(&b->d_sctl, b->nslots));
   HIPCHK(c, A.alloc(&b->d_ctl, BC_WORDS));
   HIPCHK(c, A.alloc(&b->d_wave, (size_t)pc / 64));
+  HIPCHK(c, A.alloc(&b->d_due, (size_t)b->nslots * 2));
   HIPCHK(c, A.alloc(&b->d_tri, (size_t)b->tcap));
   HIPCHK(c, A.alloc(&b->d_tbox, (size_t)b->tcap * 6));
   HIPCHK(c, A.alloc(&b->d_wc_off, (size_t)b->ccap));
@@ -497,7 +500,7 @@ int gdem_batch_create(gdem_ctx* c, const gdem_batch_params* bp, gdem_batch** out
   HIPCHK(c, hipEventCreate(&b->up));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   b->bytes_total = pc * b->bytes_per_particle + b->ccap * kCellBytes + b->tcap * kTriBytes + b->wcap * 4 +
-                   (int64_t)b->nslots * (sizeof(BScene) + sizeof(SlotCtl)) + pc / 64 * 4;
+                   (int64_t)b->nslots * (sizeof(BScene) + sizeof(SlotCtl) + 2 * 4) + pc / 64 * 4;
   *out = b.release();
   return GDEM_OK;
 }
@@ -635,12 +638,16 @@ int gdem_batch_admit(gdem_batch* b, const gdem_particles_desc* pd, const gdem_me
   }
   rc |= b->precision == GDEM_FP64 ? upload_particles<double>(b, sl, pd) : upload_particles<float>(b, sl, pd);
   SlotCtl ctl{};
-  ctl.fl[(b->step + 1) & 1] = 1;  // the lists are built by the first batch step's rebuild phase
   rc |= upload(b, b->d_sctl + sid, &ctl, sizeof(ctl));
-  const int one = 1;
-  rc |= upload(b, b->d_ctl + BC_ANY0 + ((b->step + 1) & 1), &one, sizeof(int));
   std::vector<int> wave((size_t)(sl.nseg / kSlotGranule), sid);
   rc |= upload(b, b->d_wave + sl.seg0 / kSlotGranule, wave.data(), wave.size() * 4);
+  if (!rc) {
+    // the lists are built by the first batch step's rebuild phase
+    const int par = (int)((b->step + 1) & 1);
+    if (b->precision == GDEM_FP64) launch_batch_admit<double>(b->b64, sid, par, c->stream);
+    else launch_batch_admit<float>(b->b32, sid, par, c->stream);
+    if (hipGetLastError() != hipSuccess) rc |= GDEM_E_HIP;
+  }
   if (rc) {
     (void)hipStreamSynchronize(c->stream);
     undo();

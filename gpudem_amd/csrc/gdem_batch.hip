@@ -3,8 +3,9 @@
 //
 // Every scene owns a 64-aligned segment of slots, so each wave serves exactly
 // one scene and reads that scene's constants with scalar loads.  A step is
-//   kb_bin -> kb_scan -> kb_permute -> kb_lists   (only scenes whose refresh
-//                                                  flag is set do any work)
+//   kb_bin -> kb_scan -> kb_permute -> kb_lists   (only the scenes on the
+//                                                  due list, whose refresh
+//                                                  flag is set, do any work)
 //   kb_step                                       (forces + integration of
 //                                                  every running scene)
 // and is identical for every step, so a run of steps is captured once into a
@@ -16,6 +17,8 @@
 // is bit-identical to the same scene stepped alone.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gdem_batch.h"
 #include "gdem_device.h"
 
@@ -26,21 +29,82 @@ namespace gdem {
 // (STEP+1)&1.
 __device__ __forceinline__ int rebuild_parity(int step) { return (step + 1) & 1; }
 
-// Scene of the wave holding slot i if that scene is due for a rebuild, else -1.
+// Entries of the due-scene list that the rebuild in progress reads.
 template <class H>
-__device__ __forceinline__ int due_scene(const BBufs<H>& B, int i, int step) {
-  const int w = __builtin_amdgcn_readfirstlane(i >> 6);
-  if (w >= (B.cap >> 6)) return -1;
-  const int sid = __builtin_amdgcn_readfirstlane(B.wave_scene[w]);
-  if (sid < 0) return -1;
-  if (__builtin_amdgcn_readfirstlane(B.sctl[sid].fl[rebuild_parity(step)]) == 0) return -1;
-  return sid;
+__device__ __forceinline__ int due_count(const BBufs<H>& B, int step) {
+  return __builtin_amdgcn_readfirstlane(B.ctl[BC_NDUE0 + rebuild_parity(step)]);
 }
 
-template <class H>
-__device__ __forceinline__ bool any_due(const BBufs<H>& B, int step) {
-  return __builtin_amdgcn_readfirstlane(B.ctl[BC_ANY0 + rebuild_parity(step)]) != 0;
+// The rebuild kernels run on a fixed grid and visit the waves of the due
+// scenes only.  The due scenes' waves, concatenated in list order, are the
+// work items; item g goes to wave g mod (waves of the grid).  Each workgroup
+// takes the list DUE_CHUNK entries at a time, with their wave counts and
+// prefix sums in LDS, and finds an item's entry by binary search.  An entry
+// whose flag was cleared since it was listed (an evicted scene) has no
+// waves.  f(sid, slot) runs with the whole wave active and must not
+// synchronise the workgroup.
+constexpr int DUE_CHUNK = 256;
+struct DueLds {
+  int sid[DUE_CHUNK];
+  int pre[DUE_CHUNK + 1];  // waves of the chunk's entries before entry j
+};
+
+template <class H, class F>
+__device__ __forceinline__ void for_due_waves(const BBufs<H>& B, int step, DueLds& d, F&& f) {
+  constexpr int PER = DUE_CHUNK / 64;
+  const int par = rebuild_parity(step);
+  const int nd = due_count(B, step);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (tid >> 6);
+  const int GW = gridDim.x * (blockDim.x >> 6);
+  int before = 0;  // items of the earlier chunks
+  for (int j0 = 0; j0 < nd; j0 += DUE_CHUNK) {
+    if (tid < 64) {
+      int w[PER], sum = 0;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int j = j0 + tid * PER + u;
+        int sid = -1;
+        w[u] = 0;
+        if (j < nd) {
+          sid = B.due[par * B.nslots + j];
+          if (B.sctl[sid].fl[par] != 0) w[u] = B.scenes[sid].nseg >> 6;
+          else sid = -1;
+        }
+        d.sid[tid * PER + u] = sid;
+        sum += w[u];
+      }
+      int run = wave_incl_sum(sum) - sum;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        d.pre[tid * PER + u] = run;
+        run += w[u];
+      }
+      if (tid == 63) d.pre[DUE_CHUNK] = run;
+    }
+    __syncthreads();
+    const int tot = d.pre[DUE_CHUNK];
+    for (int g = before + ((gw - before) % GW + GW) % GW; g < before + tot; g += GW) {
+      const int local = g - before;
+      int lo = 0, hi = DUE_CHUNK;  // d.pre[lo] <= local < d.pre[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (d.pre[mid] <= local) lo = mid;
+        else hi = mid;
+      }
+      const int sid = __builtin_amdgcn_readfirstlane(d.sid[lo]);
+      const int w0 = B.scenes[sid].seg0 >> 6;
+      f(sid, (w0 + local - d.pre[lo]) * 64 + lane);
+    }
+    before += tot;
+    __syncthreads();  // every wave is done with d before the next chunk
+  }
 }
+
+// Grids of the rebuild kernels (they walk the due waves, so a fixed grid
+// serves any number of due scenes).
+constexpr int RB_GRID = 1024;
+constexpr int RB_SCAN_GRID = 512;
 
 // Cell of every slot of a due scene (d_calcCellId, BoundingBox.cu:1089-1104)
 // and its rank among the particles of that cell.  Inactive particles, and any
@@ -48,113 +112,118 @@ __device__ __forceinline__ bool any_due(const BBufs<H>& B, int step) {
 template <class H>
 __global__ __launch_bounds__(256) void kb_bin(BBufs<H> B) {
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (!any_due(B, step)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sid = due_scene(B, i, step);
-  if (sid < 0) return;
-  const BScene& S = B.scenes[sid];
-  SlotCtl& C = B.sctl[sid];
-  const int cur = (step & 1) ^ __builtin_amdgcn_readfirstlane(C.q);
-  const double4 p = B.pos[cur][i];
-  bool in = p.w >= 0.;
-  int c = -1;
-  if (in) {
-    const double fx = floor((p.x - S.bmin[0]) * S.invd[0]) + 1;
-    const double fy = floor((p.y - S.bmin[1]) * S.invd[1]) + 1;
-    const double fz = floor((p.z - S.bmin[2]) * S.invd[2]) + 1;
-    in = fx >= 0. && fx < (double)S.size[0] && fy >= 0. && fy < (double)S.size[1] && fz >= 0. &&
-         fz < (double)S.size[2];
-    if (in) c = S.cell_base + (S.size[1] * (int)fz + (int)fy) * S.size[0] + (int)fx;
-  }
-  if (in) {
-    B.bin_cell[i] = c;
-    B.bin_rank[i] = atomicAdd(&B.ccount[c], 1);
-  } else {
-    B.bin_cell[i] = -1;
-    B.bin_rank[i] = atomicAdd(&C.inact, 1);
-  }
-  // this wave's particles per chunk of the scene's cells (each workgroup of
-  // the scan starts from the particles of the chunks before its own)
-  const int cc = (S.ncell + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
-  const int ch = in ? (c - S.cell_base) / cc : -1;
+  if (due_count(B, step) == 0) return;
+  __shared__ DueLds due;
+  for_due_waves(B, step, due, [&](const int sid, const int i) {
+    const BScene& S = B.scenes[sid];
+    SlotCtl& C = B.sctl[sid];
+    const int cur = (step & 1) ^ __builtin_amdgcn_readfirstlane(C.q);
+    const double4 p = B.pos[cur][i];
+    bool in = p.w >= 0.;
+    int c = -1;
+    if (in) {
+      const double fx = floor((p.x - S.bmin[0]) * S.invd[0]) + 1;
+      const double fy = floor((p.y - S.bmin[1]) * S.invd[1]) + 1;
+      const double fz = floor((p.z - S.bmin[2]) * S.invd[2]) + 1;
+      in = fx >= 0. && fx < (double)S.size[0] && fy >= 0. && fy < (double)S.size[1] && fz >= 0. &&
+           fz < (double)S.size[2];
+      if (in) c = S.cell_base + (S.size[1] * (int)fz + (int)fy) * S.size[0] + (int)fx;
+    }
+    if (in) {
+      B.bin_cell[i] = c;
+      B.bin_rank[i] = atomicAdd(&B.ccount[c], 1);
+    } else {
+      B.bin_cell[i] = -1;
+      B.bin_rank[i] = atomicAdd(&C.inact, 1);
+    }
+    // this wave's particles per chunk of the scene's cells (each workgroup of
+    // the scan starts from the particles of the chunks before its own)
+    const int cc = (S.ncell + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+    const int ch = in ? (c - S.cell_base) / cc : -1;
 #pragma unroll
-  for (int k = 0; k < SCAN_CHUNKS; ++k) {
-    const int m = __popcll(__ballot(ch == k));
-    if (m && __lane_id() == 0) atomicAdd(&C.chunk[k], m);
-  }
+    for (int k = 0; k < SCAN_CHUNKS; ++k) {
+      const int m = __popcll(__ballot(ch == k));
+      if (m && __lane_id() == 0) atomicAdd(&C.chunk[k], m);
+    }
+  });
 }
 
-// SCAN_CHUNKS workgroups per due scene, each a contiguous chunk of its cells:
-// exclusive scan of the cell counts into the (first, end) slot table,
+// SCAN_CHUNKS workgroup tasks per due scene, each a contiguous chunk of its
+// cells: exclusive scan of the cell counts into the (first, end) slot table,
 // starting from the particles the binning pass counted in the chunks before
-// (no workgroup waits for another), clearing the counters for the next
-// rebuild.  The last chunk's workgroup records the scene's active count.
+// (no task waits for another), clearing the counters for the next rebuild.
+// The last chunk's task records the scene's active count.  The tasks of
+// the due scenes are spread over a fixed grid.
 constexpr int SCAN_PER = 16;
 
 template <class H>
 __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
-  const int step = B.ctl[BC_STEP];
-  if (B.ctl[BC_ANY0 + rebuild_parity(step)] == 0) return;
-  const int sid = blockIdx.x / SCAN_CHUNKS, part_k = blockIdx.x % SCAN_CHUNKS;
-  SlotCtl& C = B.sctl[sid];
-  if (C.fl[rebuild_parity(step)] == 0) return;
-  const BScene& S = B.scenes[sid];
+  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
+  const int nd = due_count(B, step);
+  const int par = rebuild_parity(step);
   // per round: each thread sums SCAN_PER counts, each wave scans its 64 sums
   // with lane shuffles, and the four wave totals are combined through LDS
   // (two workgroup barriers per round)
   __shared__ int part[4];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int base = S.cell_base, n = S.ncell;
-  const int cc = (n + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
-  const int lo = part_k * cc, hi = min(n, lo + cc);
-  int carry = 0;
-  for (int k = 0; k < part_k; ++k) carry += C.chunk[k];
-  // the next round's counts are requested before this round's stores, so
-  // their latency overlaps the scan
-  int v[SCAN_PER];
-#pragma unroll
-  for (int k = 0; k < SCAN_PER; ++k) {
-    const int x = lo + tid * SCAN_PER + k;
-    v[k] = x < hi ? B.ccount[base + x] : 0;
-  }
-  for (int c0 = lo; c0 < hi; c0 += 256 * SCAN_PER) {
-    const int beg = c0 + tid * SCAN_PER;
-    const int nbeg = beg + 256 * SCAN_PER;
-    int vn[SCAN_PER];
-#pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) vn[k] = (nbeg + k < hi) ? B.ccount[base + nbeg + k] : 0;
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) sum += v[k];
-    const int incl = wave_incl_sum(sum);
-    if (lane == 63) part[wave] = incl;
-    __syncthreads();
-    int before = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const int t = part[w];
-      if (w < wave) before += t;
-      total += t;
-    }
-    int run = S.seg0 + carry + before + incl - sum;
-    __syncthreads();  // every wave has read part[] before the next round writes it
+  for (int task = blockIdx.x; task < nd * SCAN_CHUNKS; task += gridDim.x) {
+    const int sid = __builtin_amdgcn_readfirstlane(B.due[par * B.nslots + task / SCAN_CHUNKS]);
+    const int part_k = task % SCAN_CHUNKS;
+    SlotCtl& C = B.sctl[sid];
+    if (__builtin_amdgcn_readfirstlane(C.fl[par]) == 0) continue;
+    const BScene& S = B.scenes[sid];
+    const int base = S.cell_base, n = S.ncell;
+    const int cc = (n + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
+    const int lo = part_k * cc, hi = min(n, lo + cc);
+    int carry = 0;
+    for (int k = 0; k < part_k; ++k) carry += C.chunk[k];
+    // the next round's counts are requested before this round's stores, so
+    // their latency overlaps the scan
+    int v[SCAN_PER];
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) {
-      if (beg + k < hi) {
-        B.cell_tab[base + beg + k] = make_int2(run, run + v[k]);
-        if (v[k]) B.ccount[base + beg + k] = 0;
-        run += v[k];
-      }
+      const int x = lo + tid * SCAN_PER + k;
+      v[k] = x < hi ? B.ccount[base + x] : 0;
     }
-    carry += total;
+    for (int c0 = lo; c0 < hi; c0 += 256 * SCAN_PER) {
+      const int beg = c0 + tid * SCAN_PER;
+      const int nbeg = beg + 256 * SCAN_PER;
+      int vn[SCAN_PER];
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
-  }
-  if (tid == 0 && part_k == SCAN_CHUNKS - 1) {
-    C.nact = carry;
-    C.inact = 0;
-    C.q2 = C.q ^ 1;
+      for (int k = 0; k < SCAN_PER; ++k) vn[k] = (nbeg + k < hi) ? B.ccount[base + nbeg + k] : 0;
+      int sum = 0;
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; ++k) sum += v[k];
+      const int incl = wave_incl_sum(sum);
+      if (lane == 63) part[wave] = incl;
+      __syncthreads();
+      int before = 0, total = 0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int t = part[w];
+        if (w < wave) before += t;
+        total += t;
+      }
+      int run = S.seg0 + carry + before + incl - sum;
+      __syncthreads();  // every wave has read part[] before the next round writes it
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; ++k) {
+        if (beg + k < hi) {
+          B.cell_tab[base + beg + k] = make_int2(run, run + v[k]);
+          if (v[k]) B.ccount[base + beg + k] = 0;
+          run += v[k];
+        }
+      }
+      carry += total;
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; ++k) v[k] = vn[k];
+    }
+    if (tid == 0 && part_k == SCAN_CHUNKS - 1) {
+      C.nact = carry;
+      C.inact = 0;
+      C.q2 = C.q ^ 1;
+    }
   }
 }
 
@@ -164,46 +233,46 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
 template <class H>
 __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (!any_due(B, step)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sid = due_scene(B, i, step);
-  if (sid < 0) return;
-  const BScene& S = B.scenes[sid];
-  SlotCtl& C = B.sctl[sid];
-  const int q = __builtin_amdgcn_readfirstlane(C.q);
-  const int cur = (step & 1) ^ q;
-  // new slot of slot j of this scene
-  auto dest = [&](int j) {
-    const int c = B.bin_cell[j];
-    const int r = B.bin_rank[j];
-    return c >= 0 ? B.cell_tab[c].x + r : S.seg0 + C.nact + r;
-  };
-  const int c = B.bin_cell[i];
-  const int d = dest(i);
-  if (d < S.seg0 || d >= S.seg0 + S.nseg) {
-    atomicOr(&C.err, GDEM_ERR_NB_OVERFLOW_BIT);  // inconsistent counts: never write outside the segment
-    return;
-  }
-  const size_t N = (size_t)B.cap;
-  B.pos[cur ^ 1][d] = B.pos[cur][i];
-  B.vel[cur ^ 1][d] = B.vel[cur][i];
-  B.angv[cur ^ 1][d] = B.angv[cur][i];
-  B.mat[q ^ 1][d] = B.mat[q][i];
-  B.pid[q ^ 1][d] = B.pid[q][i];
-  B.cell_of[d] = c;
-  const int nh = B.nh[q][i];
-  B.nh[q ^ 1][d] = nh;
-  for (int k = 0; k < nh; ++k) {
-    const int j = B.hkey[q][k * N + i];  // the partner's old slot, in this scene
-    B.hkey[q ^ 1][k * N + d] = (j >= S.seg0 && j < S.seg0 + S.nseg) ? dest(j) : -1;
-    B.hd[q ^ 1][k * N + d] = B.hd[q][k * N + i];
-  }
-  const int nw = B.nw[q][i];
-  B.nw[q ^ 1][d] = nw;
-  for (int k = 0; k < nw; ++k) {
-    B.wkey[q ^ 1][k * N + d] = B.wkey[q][k * N + i];
-    B.wd[q ^ 1][k * N + d] = B.wd[q][k * N + i];
-  }
+  if (due_count(B, step) == 0) return;
+  __shared__ DueLds due;
+  for_due_waves(B, step, due, [&](const int sid, const int i) {
+    const BScene& S = B.scenes[sid];
+    SlotCtl& C = B.sctl[sid];
+    const int q = __builtin_amdgcn_readfirstlane(C.q);
+    const int cur = (step & 1) ^ q;
+    // new slot of slot j of this scene
+    auto dest = [&](int j) {
+      const int c = B.bin_cell[j];
+      const int r = B.bin_rank[j];
+      return c >= 0 ? B.cell_tab[c].x + r : S.seg0 + C.nact + r;
+    };
+    const int c = B.bin_cell[i];
+    const int d = dest(i);
+    if (d < S.seg0 || d >= S.seg0 + S.nseg) {
+      atomicOr(&C.err, GDEM_ERR_NB_OVERFLOW_BIT);  // inconsistent counts: never write outside the segment
+      return;
+    }
+    const size_t N = (size_t)B.cap;
+    B.pos[cur ^ 1][d] = B.pos[cur][i];
+    B.vel[cur ^ 1][d] = B.vel[cur][i];
+    B.angv[cur ^ 1][d] = B.angv[cur][i];
+    B.mat[q ^ 1][d] = B.mat[q][i];
+    B.pid[q ^ 1][d] = B.pid[q][i];
+    B.cell_of[d] = c;
+    const int nh = B.nh[q][i];
+    B.nh[q ^ 1][d] = nh;
+    for (int k = 0; k < nh; ++k) {
+      const int j = B.hkey[q][k * N + i];  // the partner's old slot, in this scene
+      B.hkey[q ^ 1][k * N + d] = (j >= S.seg0 && j < S.seg0 + S.nseg) ? dest(j) : -1;
+      B.hd[q ^ 1][k * N + d] = B.hd[q][k * N + i];
+    }
+    const int nw = B.nw[q][i];
+    B.nw[q ^ 1][d] = nw;
+    for (int k = 0; k < nw; ++k) {
+      B.wkey[q ^ 1][k * N + d] = B.wkey[q][k * N + i];
+      B.wd[q ^ 1][k * N + d] = B.wd[q][k * N + i];
+    }
+  });
 }
 
 // Pair and wall candidate lists of the permuted scene (the single-scene list
@@ -212,53 +281,70 @@ template <class H, bool LONG>
 __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   __shared__ PairLds lds;
   const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (!any_due(B, step)) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sid = due_scene(B, i, step);
-  if (sid < 0) return;
-  const BScene& S = B.scenes[sid];
-  SlotCtl& C = B.sctl[sid];
-  const int qn = __builtin_amdgcn_readfirstlane(C.q2);
-  const int cur = (step & 1) ^ qn;
-  ListArgs L;
-  L.pos = B.pos[cur];
-  L.pid = B.pid[qn];
-  L.cell_of = B.cell_of;
-  L.cell_tab = B.cell_tab;
-  L.nbr = B.nbr;
-  L.nnb = B.nnb;
-  L.ref = B.ref;
-  L.angv = B.angv[cur];
-  L.wl = B.wl;
-  L.nwl = B.nwl;
-  L.tri = B.tri;
-  L.bvh = nullptr;
-  L.tbox = B.tbox;
-  L.wc_off = B.wc_off;
-  L.wc_tri = B.wc_tri;
-  L.err = &C.err;
-  L.N = B.cap;
-  for (int a = 0; a < 3; ++a) L.size[a] = S.size[a];
-  L.cell_base = S.cell_base;
-  L.skin = S.skin;
-  for (int a = 0; a < 3; ++a) {
-    L.qlo[a] = S.bmin[a];
-    L.qhi[a] = S.bmax[a];
+  if (due_count(B, step) == 0) return;
+  __shared__ DueLds due;
+  for_due_waves(B, step, due, [&](const int sid, const int i) {
+    const BScene& S = B.scenes[sid];
+    SlotCtl& C = B.sctl[sid];
+    const int qn = __builtin_amdgcn_readfirstlane(C.q2);
+    const int cur = (step & 1) ^ qn;
+    ListArgs L;
+    L.pos = B.pos[cur];
+    L.pid = B.pid[qn];
+    L.cell_of = B.cell_of;
+    L.cell_tab = B.cell_tab;
+    L.nbr = B.nbr;
+    L.nnb = B.nnb;
+    L.ref = B.ref;
+    L.angv = B.angv[cur];
+    L.wl = B.wl;
+    L.nwl = B.nwl;
+    L.tri = B.tri;
+    L.bvh = nullptr;
+    L.tbox = B.tbox;
+    L.wc_off = B.wc_off;
+    L.wc_tri = B.wc_tri;
+    L.err = &C.err;
+    L.N = B.cap;
+    for (int a = 0; a < 3; ++a) L.size[a] = S.size[a];
+    L.cell_base = S.cell_base;
+    L.skin = S.skin;
+    for (int a = 0; a < 3; ++a) {
+      L.qlo[a] = S.bmin[a];
+      L.qhi[a] = S.bmax[a];
+    }
+    double qeps;
+    refq_set(L.qlo, L.qhi, L.qh, qeps);
+    L.nb_cap = B.nb_cap;
+    L.wall_cap = B.wall_cap;
+    L.ntri = S.ntri;
+    L.wall_table = 1;
+    const bool active = (i - S.seg0) < C.nact;
+    if (i == S.seg0)
+      for (int k = 0; k < SCAN_CHUNKS; ++k) C.chunk[k] = 0;  // read by this rebuild's scan only
+    pairlist_one<PL_BLOCK, LONG>(L, i, active, lds);
+    walllist_one<false>(L, i, active);  // batch scenes always have the per-cell table
+    // every later reader of q runs in a later kernel; this kernel reads q2 only
+    if (__lane_id() == 0) C.q = qn;
+    if (i == S.seg0) C.rebuilds += 1;
+  });
+}
+
+// Admission of a scene: raises its refresh flag of parity par and lists it
+// unless the list holds it already (a scene evicted from the same slot since
+// the list was filled).  One wave; it runs in stream order after every
+// earlier batch step.
+template <class H>
+__global__ __launch_bounds__(64) void kb_admit(BBufs<H> B, int sid, int par) {
+  const int lane = threadIdx.x;
+  const int n = B.ctl[BC_NDUE0 + par];
+  bool listed = false;
+  for (int j = lane; j < n; j += 64) listed |= B.due[par * B.nslots + j] == sid;
+  if (__ballot(listed) == 0 && lane == 0) {
+    B.due[par * B.nslots + n] = sid;
+    B.ctl[BC_NDUE0 + par] = n + 1;
   }
-  double qeps;
-  refq_set(L.qlo, L.qhi, L.qh, qeps);
-  L.nb_cap = B.nb_cap;
-  L.wall_cap = B.wall_cap;
-  L.ntri = S.ntri;
-  L.wall_table = 1;
-  const bool active = (i - S.seg0) < C.nact;
-  if (i == S.seg0)
-    for (int k = 0; k < SCAN_CHUNKS; ++k) C.chunk[k] = 0;  // read by this rebuild's scan only
-  pairlist_one<PL_BLOCK, LONG>(L, i, active, lds);
-  walllist_one<false>(L, i, active);  // batch scenes always have the per-cell table
-  // every later reader of q runs in a later kernel; this kernel reads q2 only
-  if (__lane_id() == 0) C.q = qn;
-  if (i == S.seg0) C.rebuilds += 1;
+  if (lane == 0) B.sctl[sid].fl[par] = 1;
 }
 
 // Fused contact step of every running scene.
@@ -327,15 +413,18 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
                                              refresh);
       // a refresh at the scene's last step would only move its final state
       if (__ballot(refresh) != 0 && __lane_id() == 0 && step + 1 < S.end_step) {
-        C.fl[step & 1] = 1;
-        B.ctl[BC_ANY0 + (step & 1)] = 1;
+        // the first wave of the scene to raise the flag lists the scene
+        if (atomicExch(&C.fl[step & 1], 1) == 0) {
+          const int k = atomicAdd(&B.ctl[BC_NDUE0 + (step & 1)], 1);
+          B.due[(step & 1) * B.nslots + k] = sid;
+        }
       }
       if (err) atomicOr(&C.err, err);
     }
     // the flag the preceding rebuild consumed
     if ((s >> 6) == (S.seg0 >> 6) && __lane_id() == 0) C.fl[rebuild_parity(step)] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.ctl[BC_ANY0 + rebuild_parity(step)] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.ctl[BC_NDUE0 + rebuild_parity(step)] = 0;
   if (last_block(&B.ctl[BC_DONE])) {
     B.ctl[BC_DONE] = 0;
     atomicExch(&B.ctl[BC_STEP], step + 1);
@@ -402,16 +491,21 @@ constexpr int BWCAP_LDS = 8;
 template <class T, class H>
 void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st) {
   if (grid <= 0) return;
-  const int nslots = B.nslots;
-  kb_bin<H><<<grid, 256, 0, st>>>(B);
-  kb_scan<H><<<nslots * SCAN_CHUNKS, 256, 0, st>>>(B);
-  kb_permute<H><<<grid, 256, 0, st>>>(B);
-  if (B.nb_cap <= PL_LDS_CAP) kb_lists<H, false><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
-  else kb_lists<H, true><<<grid * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
+  const int rg = std::min(grid, RB_GRID), sg = std::min(B.nslots * SCAN_CHUNKS, RB_SCAN_GRID);
+  kb_bin<H><<<rg, 256, 0, st>>>(B);
+  kb_scan<H><<<sg, 256, 0, st>>>(B);
+  kb_permute<H><<<rg, 256, 0, st>>>(B);
+  if (B.nb_cap <= PL_LDS_CAP) kb_lists<H, false><<<rg * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
+  else kb_lists<H, true><<<rg * (256 / PL_BLOCK), PL_BLOCK, 0, st>>>(B);
   if (hist_lds_fits(B.cap, B.hist_cap, B.whist_cap, BPCAP_LDS, BWCAP_LDS))
     kb_step<T, H, HistLds<H, BPCAP_LDS>, HistLds<H, BWCAP_LDS>><<<grid, 256, 0, st>>>(B);
   else
     kb_step<T, H, HistScan<H>, HistScan<H>><<<grid, 256, 0, st>>>(B);
+}
+
+template <class H>
+void launch_batch_admit(const BBufs<H>& B, int sid, int par, hipStream_t st) {
+  kb_admit<H><<<1, 64, 0, st>>>(B, sid, par);
 }
 
 template <class H>
@@ -427,6 +521,8 @@ void launch_batch_counts(const BBufs<H>& B, int sid, int nseg, unsigned long lon
 
 template void launch_batch_step<double, double>(const BBufs<double>&, int, hipStream_t);
 template void launch_batch_step<float, float>(const BBufs<float>&, int, hipStream_t);
+template void launch_batch_admit<double>(const BBufs<double>&, int, int, hipStream_t);
+template void launch_batch_admit<float>(const BBufs<float>&, int, int, hipStream_t);
 template void launch_batch_download<double>(const BBufs<double>&, int, int, double*, double*, double*, int*,
                                             hipStream_t);
 template void launch_batch_download<float>(const BBufs<float>&, int, int, double*, double*, double*, int*,

@@ -168,8 +168,8 @@ struct SlotCtl {
 
 enum BCtl : int {
   BC_STEP = 0,  // batch steps completed
-  BC_ANY0 = 1,  // some scene requested a rebuild at STEP parity 0 / 1
-  BC_ANY1 = 2,
+  BC_NDUE0 = 1,  // entries of the due-scene list of STEP parity 0 / 1
+  BC_NDUE1 = 2,
   BC_DONE = 3,  // last-block counter of the step kernel
   BC_WORDS = 8
 };
@@ -205,6 +205,11 @@ struct BBufs {
   const int* wave_scene;  // scene of every wave of 64 slots, -1 if free
   const BScene* scenes;
   SlotCtl* sctl;
+  // Scenes that requested a rebuild, per STEP parity: due[p * nslots + j],
+  // j < ctl[BC_NDUE0 + p].  A scene is listed at most once per parity (its
+  // flag fl[p] guards the append); an entry whose flag has since been
+  // cleared (an evicted scene) is skipped.
+  int* due;
   int* ctl;
   int cap;  // slot stride of the slot-major arrays
   int nb_cap, wall_cap, hist_cap, whist_cap;

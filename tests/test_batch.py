@@ -122,6 +122,39 @@ def test_continuous_admission_and_eviction(ctx, precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", [GDEM_FP64, GDEM_MIXED])
+def test_readmission_while_a_rebuild_is_pending(ctx, precision):
+    # The step kernel lists every scene that requests a rebuild, for the
+    # rebuild of the next batch step.  A scene evicted right after such a
+    # request leaves its entry on that list; a scene admitted into the same
+    # scene slot before the next step must be rebuilt exactly once and end
+    # as when stepped alone, and the scene running beside them must be
+    # unaffected.  With velocities of 0.3 m/s per component and a refresh
+    # distance of 1.1e-6 m, some sphere of the evicted scene passes it in
+    # every 5e-6 s step, so the scene requests a rebuild after every step.
+    fast = mk(21, 1500, skin_factor=0.002)
+    fast.v = np.random.default_rng(21).normal(0.0, 0.3, size=(fast.n, 3))
+    other, newc = mk(22, 2000, mu=0.7), mk(23, 1200, cor=0.8)
+    say(f"re-admission with a pending rebuild, precision {precision}")
+    b = Batch(ctx, precision=precision, max_scenes=2, particle_capacity=8192, graph_steps=0, **CAPS)
+    ia = admit(b, fast, 400)
+    ib = admit(b, other, 700)
+    b.step(300)
+    st = b.status(ia)
+    # the admission's build and one for each of steps 0..298; step 299's
+    # request is the pending one
+    assert st["errors"] == 0 and st["rebuilds"] == 300, st
+    b.evict(ia)
+    ic = admit(b, newc, 400)
+    assert ic == ia  # the only free scene slot
+    b.step(400)
+    assert sorted(b.poll()) == sorted([ib, ic])
+    assert b.status(ic)["errors"] == 0 and b.status(ib)["errors"] == 0
+    assert_same(b.download(ic), alone(ctx, newc, precision, 400)[0], "re-admitted")
+    assert_same(b.download(ib), alone(ctx, other, precision, 700)[0], "neighbour")
+    b.close()
+
+
 def test_skin_and_cell_do_not_change_results(ctx):
     # lists are supersets of the contacts whenever refresh <= skin/2, so the
     # tuner's skin/cell choice never changes a trajectory
