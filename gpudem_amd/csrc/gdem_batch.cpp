@@ -109,7 +109,7 @@ struct gdem_batch {
   gdem::SlotCtl* d_sctl = nullptr;
   int* d_ctl = nullptr;
   int* d_wave = nullptr;
-  int* d_due = nullptr;
+  int4* d_due = nullptr;
   gdem::TriGeo* d_tri = nullptr;
   double* d_tbox = nullptr;
   int* d_wc_off = nullptr;
@@ -500,7 +500,7 @@ int gdem_batch_create(gdem_ctx* c, const gdem_batch_params* bp, gdem_batch** out
   HIPCHK(c, hipEventCreate(&b->up));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   b->bytes_total = pc * b->bytes_per_particle + b->ccap * kCellBytes + b->tcap * kTriBytes + b->wcap * 4 +
-                   (int64_t)b->nslots * (sizeof(BScene) + sizeof(SlotCtl) + 2 * 4) + pc / 64 * 4;
+                   (int64_t)b->nslots * (sizeof(BScene) + sizeof(SlotCtl) + 2 * 16) + pc / 64 * 4;
   *out = b.release();
   return GDEM_OK;
 }
@@ -748,6 +748,12 @@ int gdem_batch_evict(gdem_batch* b, int sid) {
   int rc = 0;
   rc |= upload(b, b->d_wave + sl.seg0 / kSlotGranule, wave.data(), wave.size() * 4);
   rc |= upload(b, b->d_sctl + sid, &ctl, sizeof(ctl));
+  // a rebuild the scene requested at its last step must not run
+  if (!rc) {
+    if (b->precision == GDEM_FP64) launch_batch_evict<double>(b->b64, sid, c->stream);
+    else launch_batch_evict<float>(b->b32, sid, c->stream);
+    if (hipGetLastError() != hipSuccess) rc |= GDEM_E_HIP;
+  }
   if (rc) return fail(c, GDEM_E_HIP, "batch: eviction upload failed");
   HIPCHK(c, hipEventRecord(b->up, c->stream));
   b->parts.release(sl.seg0, sl.nseg);

@@ -14,6 +14,9 @@ void launch_batch_step(const BBufs<H>& B, int grid, hipStream_t st);
 // that parity's due list (in stream order after the earlier batch steps).
 template <class H>
 void launch_batch_admit(const BBufs<H>& B, int sid, int par, hipStream_t st);
+// Scrubs an evicted scene's entries from both due lists.
+template <class H>
+void launch_batch_evict(const BBufs<H>& B, int sid, hipStream_t st);
 template <class H>
 void launch_batch_download(const BBufs<H>& B, int sid, int nseg, double* x, double* v, double* w, int* act,
                            hipStream_t st);

@@ -29,31 +29,38 @@ namespace gdem {
 // (STEP+1)&1.
 __device__ __forceinline__ int rebuild_parity(int step) { return (step + 1) & 1; }
 
-// Entries of the due-scene list that the rebuild in progress reads.
+// The batch step counter and the length of the due list that the rebuild
+// in progress reads, in one load of the first four control words.
+static_assert(BC_STEP == 0 && BC_NDUE0 == 1 && BC_NDUE1 == 2, "control word layout");
+struct BatchClock {
+  int step, ndue;
+};
 template <class H>
-__device__ __forceinline__ int due_count(const BBufs<H>& B, int step) {
-  return __builtin_amdgcn_readfirstlane(B.ctl[BC_NDUE0 + rebuild_parity(step)]);
+__device__ __forceinline__ BatchClock batch_clock(const BBufs<H>& B) {
+  const int4 c = *reinterpret_cast<const int4*>(B.ctl);
+  const int step = __builtin_amdgcn_readfirstlane(c.x);
+  return {step, __builtin_amdgcn_readfirstlane(rebuild_parity(step) ? c.z : c.y)};
 }
 
 // The rebuild kernels run on a fixed grid and visit the waves of the due
 // scenes only.  The due scenes' waves, concatenated in list order, are the
 // work items; item g goes to wave g mod (waves of the grid).  Each workgroup
-// takes the list DUE_CHUNK entries at a time, with their wave counts and
-// prefix sums in LDS, and finds an item's entry by binary search.  An entry
-// whose flag was cleared since it was listed (an evicted scene) has no
-// waves.  f(sid, slot) runs with the whole wave active and must not
-// synchronise the workgroup.
+// takes the list DUE_CHUNK entries at a time, with their first waves, wave
+// counts and prefix sums in LDS, and finds an item's entry by binary search
+// (a scrubbed entry has no waves).  f(sid, slot) runs with the whole wave
+// active and must not synchronise the workgroup.
 constexpr int DUE_CHUNK = 256;
 struct DueLds {
   int sid[DUE_CHUNK];
+  int w0[DUE_CHUNK];
   int pre[DUE_CHUNK + 1];  // waves of the chunk's entries before entry j
 };
 
 template <class H, class F>
-__device__ __forceinline__ void for_due_waves(const BBufs<H>& B, int step, DueLds& d, F&& f) {
+__device__ __forceinline__ void for_due_waves(const BBufs<H>& B, const BatchClock& clk, DueLds& d, F&& f) {
   constexpr int PER = DUE_CHUNK / 64;
-  const int par = rebuild_parity(step);
-  const int nd = due_count(B, step);
+  const int4* list = B.due + (size_t)rebuild_parity(clk.step) * B.nslots;
+  const int nd = clk.ndue;
   const int tid = threadIdx.x, lane = tid & 63;
   const int gw = blockIdx.x * (blockDim.x >> 6) + (tid >> 6);
   const int GW = gridDim.x * (blockDim.x >> 6);
@@ -64,15 +71,11 @@ __device__ __forceinline__ void for_due_waves(const BBufs<H>& B, int step, DueLd
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int j = j0 + tid * PER + u;
-        int sid = -1;
-        w[u] = 0;
-        if (j < nd) {
-          sid = B.due[par * B.nslots + j];
-          if (B.sctl[sid].fl[par] != 0) w[u] = B.scenes[sid].nseg >> 6;
-          else sid = -1;
-        }
-        d.sid[tid * PER + u] = sid;
-        sum += w[u];
+        const int4 e = j < nd ? list[j] : make_int4(-1, 0, 0, 0);
+        d.sid[tid * PER + u] = e.x;
+        d.w0[tid * PER + u] = e.y;
+        w[u] = e.z;
+        sum += e.z;
       }
       int run = wave_incl_sum(sum) - sum;
 #pragma unroll
@@ -93,7 +96,7 @@ __device__ __forceinline__ void for_due_waves(const BBufs<H>& B, int step, DueLd
         else hi = mid;
       }
       const int sid = __builtin_amdgcn_readfirstlane(d.sid[lo]);
-      const int w0 = B.scenes[sid].seg0 >> 6;
+      const int w0 = __builtin_amdgcn_readfirstlane(d.w0[lo]);
       f(sid, (w0 + local - d.pre[lo]) * 64 + lane);
     }
     before += tot;
@@ -111,10 +114,11 @@ constexpr int RB_SCAN_GRID = 512;
 // position outside the grid, are ranked among the scene's inactive ones.
 template <class H>
 __global__ __launch_bounds__(256) void kb_bin(BBufs<H> B) {
-  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (due_count(B, step) == 0) return;
+  const BatchClock clk = batch_clock(B);
+  if (clk.ndue == 0) return;
+  const int step = clk.step;
   __shared__ DueLds due;
-  for_due_waves(B, step, due, [&](const int sid, const int i) {
+  for_due_waves(B, clk, due, [&](const int sid, const int i) {
     const BScene& S = B.scenes[sid];
     SlotCtl& C = B.sctl[sid];
     const int cur = (step & 1) ^ __builtin_amdgcn_readfirstlane(C.q);
@@ -158,9 +162,9 @@ constexpr int SCAN_PER = 16;
 
 template <class H>
 __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
-  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  const int nd = due_count(B, step);
-  const int par = rebuild_parity(step);
+  const BatchClock clk = batch_clock(B);
+  const int nd = clk.ndue;
+  const int4* list = B.due + (size_t)rebuild_parity(clk.step) * B.nslots;
   // per round: each thread sums SCAN_PER counts, each wave scans its 64 sums
   // with lane shuffles, and the four wave totals are combined through LDS
   // (two workgroup barriers per round)
@@ -168,10 +172,11 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   for (int task = blockIdx.x; task < nd * SCAN_CHUNKS; task += gridDim.x) {
-    const int sid = __builtin_amdgcn_readfirstlane(B.due[par * B.nslots + task / SCAN_CHUNKS]);
+    const int4 e = list[task / SCAN_CHUNKS];
+    if (__builtin_amdgcn_readfirstlane(e.z) == 0) continue;  // scrubbed: the scene was evicted
+    const int sid = __builtin_amdgcn_readfirstlane(e.x);
     const int part_k = task % SCAN_CHUNKS;
     SlotCtl& C = B.sctl[sid];
-    if (__builtin_amdgcn_readfirstlane(C.fl[par]) == 0) continue;
     const BScene& S = B.scenes[sid];
     const int base = S.cell_base, n = S.ncell;
     const int cc = (n + SCAN_CHUNKS - 1) / SCAN_CHUNKS;
@@ -232,10 +237,11 @@ __global__ __launch_bounds__(256) void kb_scan(BBufs<H> B) {
 // BoundingBox.cu:928-1005, here including history rows, stiffness and ids).
 template <class H>
 __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
-  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (due_count(B, step) == 0) return;
+  const BatchClock clk = batch_clock(B);
+  if (clk.ndue == 0) return;
+  const int step = clk.step;
   __shared__ DueLds due;
-  for_due_waves(B, step, due, [&](const int sid, const int i) {
+  for_due_waves(B, clk, due, [&](const int sid, const int i) {
     const BScene& S = B.scenes[sid];
     SlotCtl& C = B.sctl[sid];
     const int q = __builtin_amdgcn_readfirstlane(C.q);
@@ -280,10 +286,11 @@ __global__ __launch_bounds__(256) void kb_permute(BBufs<H> B) {
 template <class H, bool LONG>
 __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   __shared__ PairLds lds;
-  const int step = __builtin_amdgcn_readfirstlane(B.ctl[BC_STEP]);
-  if (due_count(B, step) == 0) return;
+  const BatchClock clk = batch_clock(B);
+  if (clk.ndue == 0) return;
+  const int step = clk.step;
   __shared__ DueLds due;
-  for_due_waves(B, step, due, [&](const int sid, const int i) {
+  for_due_waves(B, clk, due, [&](const int sid, const int i) {
     const BScene& S = B.scenes[sid];
     SlotCtl& C = B.sctl[sid];
     const int qn = __builtin_amdgcn_readfirstlane(C.q2);
@@ -330,21 +337,39 @@ __global__ __launch_bounds__(PL_BLOCK) void kb_lists(BBufs<H> B) {
   });
 }
 
-// Admission of a scene: raises its refresh flag of parity par and lists it
-// unless the list holds it already (a scene evicted from the same slot since
-// the list was filled).  One wave; it runs in stream order after every
-// earlier batch step.
+// Admission of a scene: raises its refresh flag of parity par and lists it,
+// in a scrubbed entry if there is one, else at the end.  One wave, in stream
+// order after every earlier batch step.
 template <class H>
 __global__ __launch_bounds__(64) void kb_admit(BBufs<H> B, int sid, int par) {
   const int lane = threadIdx.x;
+  int4* list = B.due + (size_t)par * B.nslots;
   const int n = B.ctl[BC_NDUE0 + par];
-  bool listed = false;
-  for (int j = lane; j < n; j += 64) listed |= B.due[par * B.nslots + j] == sid;
-  if (__ballot(listed) == 0 && lane == 0) {
-    B.due[par * B.nslots + n] = sid;
-    B.ctl[BC_NDUE0 + par] = n + 1;
+  int at = n;  // first entry of this scene slot or scrubbed
+  for (int j = lane; j < n; j += 64) {
+    const int x = list[j].x;
+    if (x == sid || x < 0) at = min(at, j);
   }
-  if (lane == 0) B.sctl[sid].fl[par] = 1;
+  for (int off = 32; off > 0; off >>= 1) at = min(at, __shfl_xor(at, off));
+  if (lane == 0) {
+    const BScene& S = B.scenes[sid];
+    list[at] = make_int4(sid, S.seg0 >> 6, S.nseg >> 6, 0);
+    if (at == n) B.ctl[BC_NDUE0 + par] = n + 1;
+    B.sctl[sid].fl[par] = 1;
+  }
+}
+
+// Eviction: scrubs the scene's entries from both due lists, so that a
+// rebuild it requested at its last step never runs.
+template <class H>
+__global__ __launch_bounds__(64) void kb_evict(BBufs<H> B, int sid) {
+  const int lane = threadIdx.x;
+  for (int par = 0; par < 2; ++par) {
+    int4* list = B.due + (size_t)par * B.nslots;
+    const int n = B.ctl[BC_NDUE0 + par];
+    for (int j = lane; j < n; j += 64)
+      if (list[j].x == sid) list[j] = make_int4(-1, 0, 0, 0);
+  }
 }
 
 // Fused contact step of every running scene.
@@ -416,7 +441,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
         // the first wave of the scene to raise the flag lists the scene
         if (atomicExch(&C.fl[step & 1], 1) == 0) {
           const int k = atomicAdd(&B.ctl[BC_NDUE0 + (step & 1)], 1);
-          B.due[(step & 1) * B.nslots + k] = sid;
+          B.due[(size_t)(step & 1) * B.nslots + k] = make_int4(sid, S.seg0 >> 6, S.nseg >> 6, 0);
         }
       }
       if (err) atomicOr(&C.err, err);
@@ -509,6 +534,11 @@ void launch_batch_admit(const BBufs<H>& B, int sid, int par, hipStream_t st) {
 }
 
 template <class H>
+void launch_batch_evict(const BBufs<H>& B, int sid, hipStream_t st) {
+  kb_evict<H><<<1, 64, 0, st>>>(B, sid);
+}
+
+template <class H>
 void launch_batch_download(const BBufs<H>& B, int sid, int nseg, double* x, double* v, double* w, int* act,
                            hipStream_t st) {
   kb_download<H><<<(nseg + 255) / 256, 256, 0, st>>>(B, sid, x, v, w, act);
@@ -523,6 +553,8 @@ template void launch_batch_step<double, double>(const BBufs<double>&, int, hipSt
 template void launch_batch_step<float, float>(const BBufs<float>&, int, hipStream_t);
 template void launch_batch_admit<double>(const BBufs<double>&, int, int, hipStream_t);
 template void launch_batch_admit<float>(const BBufs<float>&, int, int, hipStream_t);
+template void launch_batch_evict<double>(const BBufs<double>&, int, hipStream_t);
+template void launch_batch_evict<float>(const BBufs<float>&, int, hipStream_t);
 template void launch_batch_download<double>(const BBufs<double>&, int, int, double*, double*, double*, int*,
                                             hipStream_t);
 template void launch_batch_download<float>(const BBufs<float>&, int, int, double*, double*, double*, int*,

@@ -206,10 +206,12 @@ struct BBufs {
   const BScene* scenes;
   SlotCtl* sctl;
   // Scenes that requested a rebuild, per STEP parity: due[p * nslots + j],
-  // j < ctl[BC_NDUE0 + p].  A scene is listed at most once per parity (its
-  // flag fl[p] guards the append); an entry whose flag has since been
-  // cleared (an evicted scene) is skipped.
-  int* due;
+  // j < ctl[BC_NDUE0 + p], each (scene, first wave, waves, 0).  A scene is
+  // listed at most once per parity (its flag fl[p] guards the append);
+  // eviction scrubs a scene's entries to (-1, 0, 0, 0), and admission reuses
+  // a scrubbed entry before it appends, so a list never holds more than
+  // nslots entries.
+  int4* due;
   int* ctl;
   int cap;  // slot stride of the slot-major arrays
   int nb_cap, wall_cap, hist_cap, whist_cap;

@@ -126,10 +126,10 @@ def test_continuous_admission_and_eviction(ctx, precision):
 def test_readmission_while_a_rebuild_is_pending(ctx, precision):
     # The step kernel lists every scene that requests a rebuild, for the
     # rebuild of the next batch step.  A scene evicted right after such a
-    # request leaves its entry on that list; a scene admitted into the same
-    # scene slot before the next step must be rebuilt exactly once and end
-    # as when stepped alone, and the scene running beside them must be
-    # unaffected.  With velocities of 0.3 m/s per component and a refresh
+    # request must not be rebuilt (eviction scrubs its entry); a scene
+    # admitted into the same scene slot before the next step must be rebuilt
+    # exactly once and end as when stepped alone, and the scene running
+    # beside them must be unaffected.  With velocities of 0.3 m/s per component and a refresh
     # distance of 1.1e-6 m, some sphere of the evicted scene passes it in
     # every 5e-6 s step, so the scene requests a rebuild after every step.
     fast = mk(21, 1500, skin_factor=0.002)
