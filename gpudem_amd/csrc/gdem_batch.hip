@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void kb_step(BBufs<H> B) {
   }
   const Own<H> own = load_own<false>(A, s, running);
   tile_p[threadIdx.x] = own.p;
-  stage_vw<T>(lds, threadIdx.x, own.v, own.w);
+  stage_vw<T>(lds, threadIdx.x, own.v);
   __syncthreads();
   if (sid >= 0) {
     const BScene& S = B.scenes[sid];

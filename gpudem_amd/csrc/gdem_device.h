@@ -514,41 +514,44 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // ---------------------------------------------------------------------------
 
 // PRow/WRow: history row types for pairs and walls.  Pair candidates are
-// screened first (indices and positions fetched eight at a time so their
-// latencies overlap, FP64 overlap test), then the touching ones are evaluated
-// in list order.  The occupancy bound of 4 waves per SIMD costs the FP64 form
-// a few spilled registers and pays for itself (measured on MI355X).
-// Occupancy bound of the step kernels (waves per SIMD), per precision: the
-// FP64 form's wider registers would spill at the mixed form's bound.
+// screened first (indices and positions fetched screen_batch() at a time so
+// their latencies overlap, FP64 overlap test), then the touching ones are
+// evaluated in list order.
+// Occupancy bound of the step kernels (waves per SIMD), per precision: 4 for
+// the mixed form (at most 128 VGPRs, which its layout fits without spills,
+// and 40 KB of LDS per workgroup), 3 for the FP64 form, whose wider
+// registers would spill at 4.
 #ifndef GDEM_FP64_WAVES
 #define GDEM_FP64_WAVES 3
 #endif
 #ifndef GDEM_MIXED_WAVES
-#define GDEM_MIXED_WAVES 3
+#define GDEM_MIXED_WAVES 4
 #endif
 template <class T>
 constexpr int step_waves() {
   return sizeof(T) == 8 ? GDEM_FP64_WAVES : GDEM_MIXED_WAVES;
 }
 
-// The workgroup's velocities (and, in the mixed form, spins) are staged in
-// LDS with the positions, so a touching partner inside the tile is read from
-// LDS rather than gathered from memory (where its line has usually left the
-// L2 since the workgroup read it as an own record).  The mixed form stages
-// the float4 values its force arithmetic converts them to; the FP64 form
-// stages the velocity records themselves and gathers spins from memory, as
-// its double records would not leave LDS room for both at three workgroups
-// per CU.
+// The FP64 form stages the workgroup's velocity records in LDS with the
+// positions, so a touching partner inside the tile is read from LDS rather
+// than gathered from memory (where its line has usually left the L2 since the
+// workgroup read it as an own record); it gathers spins from memory, as its
+// double records leave no LDS room for both at three workgroups per CU.  The
+// mixed form runs at four workgroups per CU instead (40 KB of LDS each, the
+// positions, history keys and stash), which leaves no room for a velocity
+// tile: it gathers partners' velocities and spins from memory.  The fourth
+// wave per SIMD is worth more than the tile (mixed force kernel on the chute:
+// 112.1 us at three waves with a float velocity and spin tile, 108.3 us at
+// four without).
 template <class T>
-constexpr bool vw_tile() {
-  return sizeof(T) == 4;
+constexpr bool v_tile() {
+  return sizeof(T) == 8;
 }
 
 // Touching pairs of a particle whose list entries (partner slots) the
 // screening pass keeps in LDS for the force pass (later ones are read again
-// from the list): 8, so that with the velocity tile a workgroup takes 48 KB
-// of LDS and three fit a CU (at 52 KB only two did: the mixed kernel took 134
-// against 121 us).
+// from the list): 8, so that a workgroup takes 48 KB of LDS in the FP64 form
+// (three fit a CU) and 40 KB in the mixed form (four fit).
 template <class T>
 constexpr int hit_stash() {
   return 8;
@@ -565,10 +568,7 @@ constexpr int ve_max() {
 }
 
 template <class T>
-struct VwTile {
-  float4 tv[256];  // vx, vy, vz, 1/m of the tile's slots
-  float4 tw[256];  // wx, wy, wz of the tile's slots
-};
+struct VwTile {};
 template <>
 struct VwTile<double> {
   double4 tv[256];  // velocity records of the tile's slots
@@ -581,21 +581,11 @@ struct StepLds : VwTile<T> {
   int hit[hit_stash<T>()][256];
 };
 
-// Stages a thread's own velocity (and spin) in the tile; t is its slot's
+// Stages a thread's own velocity in the FP64 form's tile; t is its slot's
 // place in the tile.
 template <class T, class Lds>
-__device__ __forceinline__ void stage_vw(Lds& lds, int t, const double4& v, const double4& w) {
-  if constexpr (!vw_tile<T>()) {
-    lds.tv[t] = v;
-  } else {
-    float4 a = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
-    float4 b = make_float4((float)w.x, (float)w.y, (float)w.z, 0.f);
-    // (opaque, so the force pass converts its own copies where it needs
-    // them instead of keeping these live from here)
-    asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(b.z));
-    lds.tv[t] = a;
-    lds.tw[t] = b;
-  }
+__device__ __forceinline__ void stage_vw(Lds& lds, int t, const double4& v) {
+  if constexpr (v_tile<T>()) lds.tv[t] = v;
 }
 
 // feature word u (< ve_max<T>()) of thread tx
@@ -669,25 +659,33 @@ struct StepArgs {
   double qh[3], qeps;  // packed reference position: step per axis, error bound (refq_set)
 };
 
-// pair-list entries screened per batch, and (PRE) fetched with the own records
-constexpr int SCREEN_BATCH = 8;
-constexpr int PRE_ROWS = 2 * SCREEN_BATCH;
+// Pair-list entries screened per batch, and (PRE) fetched with the own
+// records: 8 in the FP64 form, 4 in the mixed form, whose register budget at
+// four waves per SIMD holds four candidate positions in flight, not eight.
+template <class H>
+constexpr int screen_batch() {
+  return sizeof(H) == 4 ? 4 : 8;
+}
+template <class H>
+constexpr int pre_rows() {
+  return 2 * screen_batch<H>();
+}
 
 // A particle's own records: state, material, list and history lengths, and
-// (PRE) the first PRE_ROWS pair-list entries (two screening batches).  All
+// (PRE) the first pre_rows() pair-list entries (two screening batches).  All
 // loads are independent, so they share one memory round trip; lanes without
 // a particle load nothing.  The entries are fetched before the list length is
 // known: rows past the length (always inside the [nb_cap][N] array) hold
 // stale entries, which the screening pass masks.  This saves the round trips
 // that wait for the length and for the second batch's entries, at the price
-// of reading PRE_ROWS 4-byte rows whatever the length: a gain for dense
+// of reading pre_rows() 4-byte rows whatever the length: a gain for dense
 // packings (lists of about eight and longer), a loss for dilute ones.
 template <class H>
 struct Own {
   double4 p, v, w;  // w.w: the packed reference position (refq_pack)
   typename Vec4<H>::type mt;
   int nnb, nwl, nh0, nw0;
-  int nb0[PRE_ROWS];
+  int nb0[pre_rows<H>()];
 };
 
 template <bool PRE, class H>
@@ -699,7 +697,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
   o.mt = {};
   o.nnb = o.nwl = o.nh0 = o.nw0 = 0;
 #pragma unroll
-  for (int u = 0; u < PRE_ROWS; ++u) o.nb0[u] = -1;
+  for (int u = 0; u < pre_rows<H>(); ++u) o.nb0[u] = -1;
   if (valid) {
     o.p = A.pin[s];
     o.v = A.vin[s];
@@ -714,7 +712,7 @@ __device__ __forceinline__ Own<H> load_own(const StepArgs<H>& A, int s, bool val
     // for all of them before the next dependent access)
     if constexpr (PRE) {
 #pragma unroll
-      for (int u = 0; u < PRE_ROWS; ++u)
+      for (int u = 0; u < pre_rows<H>(); ++u)
         o.nb0[u] = A.nbr[(size_t)(u < A.nb_cap ? u : A.nb_cap - 1) * A.N + s];
     }
   }
@@ -904,8 +902,9 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
                                               uint32_t& err, bool& refresh) {
   using H4 = typename Vec4<H>::type;
   constexpr int HS = hit_stash<T>();
+  constexpr int SB = screen_batch<H>(), PR = pre_rows<H>();
   constexpr int VEM = ve_max<T>();
-  constexpr bool VW = vw_tile<T>();
+  constexpr bool VT = v_tile<T>();
   const int tx = threadIdx.x;
   const int lane = __lane_id();
   const int wtx = tx - lane;  // thread index of lane 0 of this wave
@@ -941,11 +940,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     // waited for here (the empty asm uses them): left to the compiler, the
     // wait for the last entry would come after the wall-list and candidate
     // position loads below, and would wait for those too.
-    static_assert(PRE_ROWS == 16, "the wait below names sixteen entries");
-    asm volatile("" ::"v"(own.nb0[0]), "v"(own.nb0[1]), "v"(own.nb0[2]), "v"(own.nb0[3]), "v"(own.nb0[4]),
-                 "v"(own.nb0[5]), "v"(own.nb0[6]), "v"(own.nb0[7]));
-    asm volatile("" ::"v"(own.nb0[8]), "v"(own.nb0[9]), "v"(own.nb0[10]), "v"(own.nb0[11]), "v"(own.nb0[12]),
-                 "v"(own.nb0[13]), "v"(own.nb0[14]), "v"(own.nb0[15]));
+#pragma unroll
+    for (int u = 0; u < PR; ++u) asm volatile("" ::"v"(own.nb0[u]));
   }
   const T k = (T)mt.x, etac = (T)mt.y, sqrtm = (T)mt.z;
   const double invmoi = (double)mt.w;
@@ -975,13 +971,13 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   // records; their positions are requested now, before the history keys
   // below, because any LDS access after those loads (these tile reads
   // included) waits for them.
-  int ev0[SCREEN_BATCH];
-  double4 cv0[SCREEN_BATCH];
+  int ev0[SB];
+  double4 cv0[SB];
   if constexpr (PRE) {
 #pragma unroll
-    for (int u = 0; u < SCREEN_BATCH; ++u) ev0[u] = u < nnb ? own.nb0[u] : -1;
+    for (int u = 0; u < SB; ++u) ev0[u] = u < nnb ? own.nb0[u] : -1;
 #pragma unroll
-    for (int u = 0; u < SCREEN_BATCH; ++u) cv0[u] = ev0[u] >= 0 ? pos_of(ev0[u]) : make_double4(0., 0., 0., -1.);
+    for (int u = 0; u < SB; ++u) cv0[u] = ev0[u] >= 0 ? pos_of(ev0[u]) : make_double4(0., 0., 0., -1.);
   }
 
   // history rows of pairs and walls
@@ -1013,7 +1009,7 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     int* stash = &lds.hit[0][tx];
     auto test = [&](const int q0, const int* ev, const double4* cv) {
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u) {
+      for (int u = 0; u < SB; ++u) {
         const double4 pj = cv[u];
         if (pj.w < 0.) continue;
         const double dx = p.x - pj.x, dy = p.y - pj.y, dz = p.z - pj.z;
@@ -1031,22 +1027,22 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       // (outside the loop, so that no wait for a later batch's loads precedes
       // it; likewise the second batch, whose entries are in registers)
       test(0, ev0, cv0);
-      int ev1[SCREEN_BATCH];
-      double4 cv1[SCREEN_BATCH];
+      int ev1[SB];
+      double4 cv1[SB];
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u) ev1[u] = SCREEN_BATCH + u < nnb ? own.nb0[SCREEN_BATCH + u] : -1;
+      for (int u = 0; u < SB; ++u) ev1[u] = SB + u < nnb ? own.nb0[SB + u] : -1;
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u) cv1[u] = ev1[u] >= 0 ? pos_of(ev1[u]) : make_double4(0., 0., 0., -1.);
-      test(SCREEN_BATCH, ev1, cv1);
-      q_start = PRE_ROWS;
+      for (int u = 0; u < SB; ++u) cv1[u] = ev1[u] >= 0 ? pos_of(ev1[u]) : make_double4(0., 0., 0., -1.);
+      test(SB, ev1, cv1);
+      q_start = PR;
     }
-    for (int q0 = q_start; q0 < nnb; q0 += SCREEN_BATCH) {
-      int ev[SCREEN_BATCH];
-      double4 cv[SCREEN_BATCH];
+    for (int q0 = q_start; q0 < nnb; q0 += SB) {
+      int ev[SB];
+      double4 cv[SB];
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u) ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : -1;
+      for (int u = 0; u < SB; ++u) ev[u] = (q0 + u < nnb) ? A.nbr[(size_t)(q0 + u) * N + s] : -1;
 #pragma unroll
-      for (int u = 0; u < SCREEN_BATCH; ++u) cv[u] = ev[u] >= 0 ? pos_of(ev[u]) : make_double4(0., 0., 0., -1.);
+      for (int u = 0; u < SB; ++u) cv[u] = ev[u] >= 0 ? pos_of(ev[u]) : make_double4(0., 0., 0., -1.);
       test(q0, ev, cv);
     }
     // the entries are read from other lanes' columns below
@@ -1080,10 +1076,11 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
     };
     // The partner's records are requested as soon as the list entry is
     // known, before the history key search (dependent LDS reads), which only
-    // the displacement read needs.  In the mixed form a partner inside the
-    // tile is read from LDS in pair_finish; its lane's memory reads go to the
-    // tile's first slot instead, one line the whole wave shares, so that the
-    // reads stay unconditional.
+    // the displacement read needs.  A partner inside the tile has its
+    // position (and, in the FP64 form, its velocity record) read from LDS in
+    // pair_finish; those memory reads of its lane go to the tile's first slot
+    // instead, one line the whole wave shares, so that the reads stay
+    // unconditional.
     auto pair_fetch = [&](const int base, PairIn& q) {
       const int g = base + lane;
       const bool has = g < total;
@@ -1115,8 +1112,8 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       q.in_t = (unsigned)(q.e - tile0) < 256u;
       const int me = q.in_t ? tile0 : q.e;
       q.pj = pin[me];
-      q.vj = vin[me];
-      q.wj = win[VW ? me : q.e];
+      q.vj = vin[VT ? me : q.e];
+      q.wj = win[q.e];
       // (keeps the compiler from sinking the loads above below the search)
       asm volatile("" ::: "memory");
       q.slot = has ? row.find_in(wtx + o, so, o_n, e) : -1;
@@ -1142,30 +1139,17 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
       double4 pj = q.pj;
       V3<T> vj{(T)q.vj.x, (T)q.vj.y, (T)q.vj.z}, wj{(T)q.wj.x, (T)q.wj.y, (T)q.wj.z};
       T invmj = (T)q.vj.w;
-      if constexpr (!VW) {
+      {
         const int te = (e - tile0) & 255;
         const double4 pl = ld_nt(tile_p + te);
-        const double4 vl = ld_nt(lds.tv + te);
-        if (q.in_t) {
-          pj = pl;
-          vj = V3<T>{vl.x, vl.y, vl.z};
-          invmj = vl.w;
+        if constexpr (VT) {
+          const double4 vl = ld_nt(lds.tv + te);
+          if (q.in_t) {
+            vj = V3<T>{vl.x, vl.y, vl.z};
+            invmj = vl.w;
+          }
         }
-      } else {
-        // The tile holds the same float values the conversions above make.
-        // Its reads follow them (the asm waits for the gathers and orders
-        // the LDS reads after it), so the converted values, not the double
-        // records, are what stays live beside them.
-        asm volatile("" ::"v"(vj.x), "v"(vj.y), "v"(vj.z), "v"(invmj), "v"(wj.x), "v"(wj.y), "v"(wj.z) : "memory");
-        const int te = (e - tile0) & 255;
-        const double4 pl = ld_nt(tile_p + te);
-        const float4 vl = ld_nt(lds.tv + te), wl = ld_nt(lds.tw + te);
-        if (q.in_t) {
-          pj = pl;
-          vj = V3<T>{vl.x, vl.y, vl.z};
-          invmj = vl.w;
-          wj = V3<T>{wl.x, wl.y, wl.z};
-        }
+        if (q.in_t) pj = pl;
       }
       // new contacts take their particle's next free slots in list order
       const uint64_t fresh = __ballot(has && slot < 0);
@@ -1462,23 +1446,15 @@ __device__ __forceinline__ void step_particle(const StepArgs<H>& A, int s, bool 
   if (!act) return;
 
   // ---- integration (k_integrate), out-of-bounds, refresh check --------------
-  // The FP64 form reads the particle's own records again here (the position
-  // from the tile, the rest from memory, where they are still cached) rather
-  // than holding them in registers through the contact passes, which would
-  // spill at its occupancy bound; the empty asm keeps the compiler from
-  // reusing the values loaded at the start.  The mixed form keeps them (one
-  // round trip fewer, measured faster).
-  double4 p1, v1, w1;
-  if constexpr (sizeof(T) == 8) {
-    asm volatile("" ::: "memory");
-    p1 = ld_nt(tile_p + (s - tile0));
-    v1 = vin[s];
-    w1 = win[s];
-  } else {
-    p1 = p;
-    v1 = v;
-    w1 = w;
-  }
+  // The particle's own records are read again here (the position from the
+  // tile, the rest from memory, where they are still cached) rather than held
+  // in registers through the contact passes, which would spill at either
+  // form's occupancy bound; the empty asm keeps the compiler from reusing the
+  // values loaded at the start.
+  asm volatile("" ::: "memory");
+  const double4 p1 = ld_nt(tile_p + (s - tile0));
+  const double4 v1 = vin[s];
+  const double4 w1 = win[s];
   const double dt = A.dt;
   const double ax = (double)f.x * invm + A.g[0];
   const double ay = (double)f.y * invm + A.g[1];

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256, step_waves<T>()) void k_step(SimParams P, Bufs
   const bool valid = s < P.N;
   const Own<H> own = load_own<PRE>(A, s, valid);
   tile_p[tt] = own.p;
-  stage_vw<T>(lds, tt, own.v, own.w);
+  stage_vw<T>(lds, tt, own.v);
   __syncthreads();
   uint32_t err = 0;
   bool refresh = false;
